@@ -1,0 +1,107 @@
+// engine.h -- host-side engine of the MI355X CCD library.
+//
+// EngineBase is the device counterpart of bsccs::AbstractModelSpecifics
+// (src/cyclops/engine/AbstractModelSpecifics.h:36-122): one method per virtual.
+// EngineT<Real> holds the per-fit state in HBM; DevData<Real> holds the design
+// matrix, shared by clones (ModelSpecifics.hpp:98-101 clones over the same data).
+#pragma once
+
+#include "common.h"
+#include "../../include/cyclops_amd.h"
+
+#include <memory>
+#include <vector>
+
+namespace ccd {
+
+// Device copy of the data (column store in the reference's layout).
+template <typename Real>
+struct DevData {
+    int model = 0;
+    int device = 0;
+    int64_t K = 0;
+    int J = 0;
+    bool hasOffset = false;
+    int64_t nnz = 0;
+    std::vector<int32_t> fmt;
+    std::vector<int64_t> colPtr, densePtr;
+    std::vector<double> y, time;          // host copies (pid construction, fixed terms)
+    std::vector<int32_t> strat;           // renumbered strata per row
+    std::vector<int32_t> stratStart;      // [nStrata + 1]
+    int64_t nStrata = 0;
+    // device
+    int32_t* dRows = nullptr;
+    Real* dVals = nullptr;
+    Real* dDense = nullptr;
+    Real* dY = nullptr;
+    int32_t* dFmt = nullptr;
+    int64_t* dColPtr = nullptr;
+    int64_t* dDensePtr = nullptr;
+    int32_t* dStrat = nullptr;
+    int32_t* dStratStart = nullptr;
+    double bytes = 0.0;
+
+    ~DevData();
+};
+
+struct KernelCounters {
+    double launches = 0, deviceMs = 0, bytes = 0, coordinates = 0;
+};
+
+class EngineBase {
+public:
+    virtual ~EngineBase() = default;
+    virtual int model() const = 0;
+    virtual int precision() const = 0;
+    virtual int64_t K() const = 0;
+    virtual int J() const = 0;
+    virtual int64_t N() const = 0;
+    virtual bool hasOffset() const = 0;
+    virtual int format(int j) const = 0;
+    virtual int64_t nonZeros(int j) const = 0;
+    virtual int device() const = 0;
+    virtual hipStream_t stream() const = 0;
+
+    virtual void setWeights(const double* w, bool useW) = 0;
+    virtual void fixedTerms(bool useW) = 0;
+    virtual void zeroXBeta() = 0;
+    virtual void axpyXBeta(double beta, int j) = 0;
+    virtual void computeXBeta(const double* beta) = 0;
+    virtual void remainingStats(bool useW) = 0;
+    virtual void numerator(int j, bool useW) = 0;
+    virtual void gradHess(int j, double* g, double* h, bool useW) = 0;
+    virtual void updateXBeta(double delta, int j, bool useW) = 0;
+    virtual double logLik(bool useW) = 0;
+    virtual double predLogLik(const double* w) = 0;
+    virtual void predEstimates(double* out, const double* w) = 0;
+    virtual double gradientObjective(bool useW) = 0;
+    virtual void getXBeta(double* out) = 0;
+    virtual void saveXBeta() = 0;
+    virtual void getXBetaSave(double* out) = 0;
+    virtual EngineBase* clone() const = 0;
+    virtual double footprintBytes() const = 0;
+    virtual void getStrata(int32_t* out) const = 0;   // renumbered strata per row
+
+    // Resident / batched fits (resident.hip).
+    virtual void fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) = 0;
+    virtual void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas,
+                          ccd_fit_result* r) = 0;
+
+    KernelCounters counters;
+    bool timing = false;
+};
+
+EngineBase* createEngine(const ccd_data_desc& d, int device);
+
+// Host CCD driver over an EngineBase: CyclicCoordinateDescent::update/findMode
+// (CyclicCoordinateDescent.cpp:586-615, 909-1216). This is the plain per-fit path.
+void fitPlain(EngineBase& eng, const ccd_fit_args& a, double* beta, ccd_fit_result& r);
+
+// Fold assignment of CrossValidationSelector (drivers/CrossValidationSelector.cpp:19-189).
+void assignFolds(int64_t K, const int32_t* pid, int fold, int selector, int64_t seed, int32_t* rowFold);
+
+// Prior step (CovariatePrior.h:186-188, 240-273, 410-415), shared by host drivers.
+double priorDelta(int type, double variance, double g, double h, double beta);
+double priorLogDensity(int type, double variance, double beta);
+
+}  // namespace ccd

@@ -1,0 +1,282 @@
+"""Python mirror of the reference's fitting interface over the C ABI.
+
+    Engine          <-> bsccs::AbstractModelSpecifics (per-coordinate calls)
+    fit_cyclops_model <-> fitCyclopsModel (R/ModelFit.R:65-293) for one fit
+    fit_batch       <-> the fold x grid fits of a cross-validation, batched
+    cross_validate  <-> CcdInterface::runCrossValidation (CcdInterface.cpp:731-793)
+
+Every call goes through libcyclops_amd.so; there is no CPU fallback: a missing
+library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .control import FitArgsHolder, create_control, create_prior, intercept_index
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcyclops_amd.so")
+_LIB = None
+
+
+class CyclopsError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcyclops_amd.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CyclopsError(f"{LIB_PATH} is missing: build it with `python -m cyclops_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        i32 = C.c_int32
+        L.ccd_last_error.restype = C.c_char_p
+        L.ccd_version.restype = C.c_char_p
+        L.ccd_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.ccd_create.argtypes = [C.POINTER(_abi.DataDesc), C.c_int, C.POINTER(vp)]
+        L.ccd_clone.argtypes = [vp, C.POINTER(vp)]
+        L.ccd_destroy.argtypes = [vp]
+        L.ccd_get_dims.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(i32), C.POINTER(C.c_int64)]
+        L.ccd_set_weights.argtypes = [vp, dp, C.c_int]
+        L.ccd_fixed_terms.argtypes = [vp, C.c_int]
+        L.ccd_compute_xbeta.argtypes = [vp, dp]
+        L.ccd_zero_xbeta.argtypes = [vp]
+        L.ccd_axpy_xbeta.argtypes = [vp, C.c_double, C.c_int]
+        L.ccd_remaining_stats.argtypes = [vp, C.c_int]
+        L.ccd_numerator.argtypes = [vp, C.c_int, C.c_int]
+        L.ccd_grad_hess.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+        L.ccd_update_xbeta.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        L.ccd_loglik.argtypes = [vp, C.c_int, dp]
+        L.ccd_pred_loglik.argtypes = [vp, dp, dp]
+        L.ccd_pred_est.argtypes = [vp, dp, dp]
+        L.ccd_gradient_objective.argtypes = [vp, C.c_int, dp]
+        L.ccd_get_xbeta.argtypes = [vp, dp]
+        L.ccd_save_xbeta.argtypes = [vp]
+        L.ccd_get_xbeta_save.argtypes = [vp, dp]
+        L.ccd_fit.argtypes = [vp, C.POINTER(_abi.FitArgs), C.c_int, dp, C.POINTER(_abi.FitResult)]
+        L.ccd_fit_batch.argtypes = [vp, C.POINTER(_abi.FitArgs), C.POINTER(_abi.BatchArgs), dp,
+                                    C.POINTER(_abi.FitResult)]
+        L.ccd_cross_validate.argtypes = [vp, C.POINTER(_abi.FitArgs), C.POINTER(_abi.CvArgs), dp, dp, dp,
+                                         C.POINTER(_abi.FitResult), C.POINTER(_abi.CvResult)]
+        L.ccd_cv_assign_folds.argtypes = [C.c_int64, C.POINTER(i32), i32, i32, C.c_int64, C.POINTER(i32)]
+        L.ccd_get_kernel_stats.argtypes = [vp, C.POINTER(_abi.KernelStats)]
+        L.ccd_reset_kernel_stats.argtypes = [vp]
+        L.ccd_estimate_footprint.argtypes = [C.POINTER(_abi.DataDesc), dp]
+        if hasattr(L, "ccd_server_start"):
+            L.ccd_server_start.argtypes = [C.POINTER(_abi.ServerConfig), C.POINTER(vp)]
+            L.ccd_server_submit.argtypes = [vp, C.POINTER(_abi.DataDesc), C.POINTER(_abi.FitArgs), dp,
+                                            C.POINTER(C.c_int64)]
+            L.ccd_server_poll.argtypes = [vp, C.c_int64, C.c_int, dp, C.POINTER(_abi.FitResult),
+                                          C.POINTER(_abi.RequestStats)]
+            L.ccd_server_stop.argtypes = [vp]
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        raise CyclopsError(f"ccd error {rc}: {lib().ccd_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().ccd_device_count(C.byref(n)))
+    return n.value
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    """Device engine for one dataset: the AbstractModelSpecifics interface."""
+
+    def __init__(self, data, precision=_abi.FP64, device=0, _handle=None):
+        self.data = data
+        self.precision = precision
+        if _handle is not None:
+            self.h = _handle
+            return
+        self._desc = data.desc(precision)
+        h = C.c_void_p()
+        check(lib().ccd_create(C.byref(self._desc), device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ccd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clone(self):
+        h = C.c_void_p()
+        check(lib().ccd_clone(self.h, C.byref(h)))
+        return Engine(self.data, self.precision, _handle=h)
+
+    def dims(self):
+        K, J, N = C.c_int64(), C.c_int32(), C.c_int64()
+        check(lib().ccd_get_dims(self.h, C.byref(K), C.byref(J), C.byref(N)))
+        return K.value, J.value, N.value
+
+    # --- per-coordinate interface -------------------------------------------
+    def set_weights(self, w=None):
+        self._w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+        check(lib().ccd_set_weights(self.h, None if self._w is None else _d(self._w), 0 if w is None else 1))
+
+    def fixed_terms(self, use_w=False):
+        check(lib().ccd_fixed_terms(self.h, int(use_w)))
+
+    def compute_xbeta(self, beta):
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        check(lib().ccd_compute_xbeta(self.h, _d(b)))
+
+    def zero_xbeta(self):
+        check(lib().ccd_zero_xbeta(self.h))
+
+    def axpy_xbeta(self, beta, j):
+        check(lib().ccd_axpy_xbeta(self.h, float(beta), int(j)))
+
+    def remaining_stats(self, use_w=False):
+        check(lib().ccd_remaining_stats(self.h, int(use_w)))
+
+    def numerator(self, j, use_w=False):
+        check(lib().ccd_numerator(self.h, int(j), int(use_w)))
+
+    def grad_hess(self, j, use_w=False):
+        g, h = C.c_double(), C.c_double()
+        check(lib().ccd_grad_hess(self.h, int(j), C.byref(g), C.byref(h), int(use_w)))
+        return g.value, h.value
+
+    def update_xbeta(self, j, delta, use_w=False):
+        check(lib().ccd_update_xbeta(self.h, int(j), float(delta), int(use_w)))
+
+    def loglik(self, use_w=False):
+        v = C.c_double()
+        check(lib().ccd_loglik(self.h, int(use_w), C.byref(v)))
+        return v.value
+
+    def pred_loglik(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        v = C.c_double()
+        check(lib().ccd_pred_loglik(self.h, _d(w), C.byref(v)))
+        return v.value
+
+    def pred_est(self, w=None):
+        out = np.zeros(self.data.K)
+        ww = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+        check(lib().ccd_pred_est(self.h, _d(out), None if ww is None else _d(ww)))
+        return out
+
+    def gradient_objective(self, use_w=False):
+        v = C.c_double()
+        check(lib().ccd_gradient_objective(self.h, int(use_w), C.byref(v)))
+        return v.value
+
+    def xbeta(self):
+        out = np.zeros(self.data.K)
+        check(lib().ccd_get_xbeta(self.h, _d(out)))
+        return out
+
+    def save_xbeta(self):
+        check(lib().ccd_save_xbeta(self.h))
+
+    def xbeta_save(self):
+        out = np.zeros(self.data.K)
+        check(lib().ccd_get_xbeta_save(self.h, _d(out)))
+        return out
+
+    # --- whole fits ------------------------------------------------------------
+    def fit(self, prior=None, control=None, weights=None, fixed=None, beta0=None, mode="plain"):
+        prior = prior or create_prior("none")
+        control = control or create_control()
+        hold = FitArgsHolder(self.data.J, prior, control, weights=weights, fixed=fixed,
+                             intercept_index=intercept_index(self.data))
+        beta = np.zeros(self.data.J) if beta0 is None else np.array(beta0, dtype=np.float64)
+        res = _abi.FitResult()
+        m = _abi.MODE_PLAIN if mode == "plain" else _abi.MODE_RESIDENT
+        check(lib().ccd_fit(self.h, C.byref(hold.args), m, _d(beta), C.byref(res)))
+        return beta, res.as_dict()
+
+    def fit_batch(self, variances, prior_type="laplace", control=None, held_out_fold=None, row_fold=None,
+                  betas0=None, exclude=()):
+        control = control or create_control()
+        n = len(variances)
+        prior = create_prior(prior_type, float(variances[0]), exclude=exclude)
+        hold = FitArgsHolder(self.data.J, prior, control, intercept_index=intercept_index(self.data))
+        var = np.ascontiguousarray(variances, dtype=np.float64)
+        hof = None if held_out_fold is None else np.ascontiguousarray(held_out_fold, dtype=np.int32)
+        rf = None if row_fold is None else np.ascontiguousarray(row_fold, dtype=np.int32)
+        b = _abi.BatchArgs()
+        b.n_fits = n
+        b.variances = _d(var)
+        b.held_out_fold = _abi.ptr(hof, C.c_int32)
+        b.row_fold = _abi.ptr(rf, C.c_int32)
+        betas = np.zeros((n, self.data.J)) if betas0 is None else np.array(betas0, dtype=np.float64)
+        betas = np.ascontiguousarray(betas)
+        results = (_abi.FitResult * n)()
+        check(lib().ccd_fit_batch(self.h, C.byref(hold.args), C.byref(b), _d(betas), results))
+        return betas, [r.as_dict() for r in results]
+
+    def cross_validate(self, prior, control, selector=None):
+        if selector is None:
+            # R/ModelFit.R:174-188: "auto" -> byPid for conditional models, byRow otherwise
+            selector = _abi.SELECT_BY_PID if self.data.model_type in (_abi.CONDITIONAL_LOGISTIC,) else \
+                _abi.SELECT_BY_ROW
+        hold = FitArgsHolder(self.data.J, prior, control, intercept_index=intercept_index(self.data))
+        cv = _abi.CvArgs()
+        cv.fold = control.fold
+        cv.grid_size = control.grid_size
+        cv.lower_limit = control.lower_limit
+        cv.upper_limit = control.upper_limit
+        cv.selector = selector
+        cv.seed = control.seed
+        grid = np.zeros(control.grid_size)
+        pred = np.zeros(control.grid_size * control.fold)
+        beta = np.zeros(self.data.J)
+        res = _abi.FitResult()
+        cvr = _abi.CvResult()
+        check(lib().ccd_cross_validate(self.h, C.byref(hold.args), C.byref(cv), _d(grid), _d(pred), _d(beta),
+                                       C.byref(res), C.byref(cvr)))
+        return {"grid": grid, "pred": pred.reshape(control.grid_size, control.fold), "beta": beta,
+                "result": res.as_dict(), "best_variance": cvr.best_variance, "best_value": cvr.best_value,
+                "seconds": cvr.seconds, "n_fits": cvr.n_fits}
+
+    def kernel_stats(self):
+        st = _abi.KernelStats()
+        check(lib().ccd_get_kernel_stats(self.h, C.byref(st)))
+        return {"launches": st.launches, "device_ms": st.device_ms, "algorithmic_bytes": st.algorithmic_bytes,
+                "coordinates": st.coordinates}
+
+    def reset_kernel_stats(self):
+        check(lib().ccd_reset_kernel_stats(self.h))
+
+
+def fit_cyclops_model(data, prior=None, control=None, weights=None, fixed=None, beta0=None,
+                      precision=_abi.FP64, mode="plain", device=0):
+    """One fit on the device: fitCyclopsModel (R/ModelFit.R:65-293)."""
+    eng = Engine(data, precision, device)
+    try:
+        return eng.fit(prior, control, weights=weights, fixed=fixed, beta0=beta0, mode=mode)
+    finally:
+        eng.close()
+
+
+def assign_folds(K, pid, fold, selector, seed):
+    out = np.zeros(K, dtype=np.int32)
+    p = None if pid is None else np.ascontiguousarray(pid, dtype=np.int32)
+    check(lib().ccd_cv_assign_folds(K, _abi.ptr(p, C.c_int32), fold, selector, seed,
+                                    out.ctypes.data_as(C.POINTER(C.c_int32))))
+    return out
