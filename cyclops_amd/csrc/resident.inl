@@ -1,19 +1,579 @@
-// resident.inl -- device-resident and batched fits (persistent kernels).
+// resident.inl -- host side of the device-resident and batched fits
+// (kFindMode in resident_kernels.h). Included by engine.hip after EngineT.
 #pragma once
+
+#include "resident_kernels.h"
+
+#include <chrono>
+#include <numeric>
+#include <cstdlib>
 
 namespace ccd {
 
+// Offset of the first entry of column j with row >= rowStart[w], relative to the
+// column start: lower_bound per (column, workgroup).
+__global__ __launch_bounds__(BLOCK) void kColOff(int J, int G, const int* __restrict__ fmt,
+                                                const int64_t* __restrict__ colPtr, const int* __restrict__ rows,
+                                                const int* __restrict__ rowStart, int* __restrict__ colOff) {
+    const int64_t total = (int64_t)J * (G + 1);
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < total; t += stride) {
+        const int j = (int)(t / (G + 1)), w = (int)(t % (G + 1));
+        const int f = fmt[j];
+        if (!(f == F_INDICATOR || f == F_SPARSE)) { colOff[t] = 0; continue; }
+        const int64_t a = colPtr[j], n = colPtr[j + 1] - a;
+        const int target = rowStart[w];
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (rows[a + mid] < target) lo = mid + 1;
+            else hi = mid;
+        }
+        colOff[t] = (int)lo;
+    }
+}
+
+// sum over column j of x*y, in total and per fold (F <= 32 folds).
 template <typename Real>
-void EngineT<Real>::releaseResident() {}
+__global__ __launch_bounds__(BLOCK) void kXjYFold(int J, int64_t K, int F, const int* __restrict__ fmt,
+                                                 const int64_t* __restrict__ colPtr,
+                                                 const int64_t* __restrict__ densePtr, const int* __restrict__ rows,
+                                                 const Real* __restrict__ vals, const Real* __restrict__ dense,
+                                                 const Real* __restrict__ y, const int* __restrict__ rowFold,
+                                                 double* __restrict__ tot, double* __restrict__ byFold) {
+    __shared__ double red[33][WAVES];
+    for (int j = blockIdx.x; j < J; j += gridDim.x) {
+        double acc[33];
+        for (int i = 0; i < 33; ++i) acc[i] = 0.0;
+        const int f = fmt[j];
+        const bool idx = (f == F_INDICATOR || f == F_SPARSE);
+        const int64_t a = idx ? colPtr[j] : 0, b = idx ? colPtr[j + 1] : K;
+        for (int64_t e = a + threadIdx.x; e < b; e += BLOCK) {
+            const int64_t k = idx ? rows[e] : e;
+            double c = (double)y[k];
+            if (f == F_SPARSE) c *= (double)vals[e];
+            if (f == F_DENSE) c *= (double)dense[densePtr[j] + e];
+            acc[32] += c;
+            if (rowFold) {
+                const int q = rowFold[k];
+                for (int i = 0; i < 32; ++i)
+                    if (i == q) acc[i] += c;
+            }
+        }
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int i = 0; i < 33; ++i) {
+            double x = acc[i];
+            for (int d = 32; d > 0; d >>= 1) x += __shfl_down(x, d, 64);
+            if (lane == 0) red[i][wid] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 33) {
+            double s = 0.0;
+            for (int wv = 0; wv < WAVES; ++wv) s += red[threadIdx.x][wv];
+            if (threadIdx.x == 32) tot[j] = s;
+            else if (threadIdx.x < F && byFold) byFold[(size_t)j * F + threadIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// xb[k*BP + f] += beta_f * x_kj for every fit f (one column).
+template <typename Real, int FMT>
+__global__ __launch_bounds__(BLOCK) void kAxpyBatch(Col<Real> c, const double* __restrict__ beta, int B, int BP,
+                                                   Real* __restrict__ xb) {
+    const int64_t total = c.n * (int64_t)B;
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < total; t += stride) {
+        const int64_t e = t / B;
+        const int f = (int)(t % B);
+        const double bf = beta[f];
+        if (bf == 0.0) continue;
+        xb[rowOf<Real, FMT>(c, e) * BP + f] += (Real)bf * xOf<Real, FMT>(c, e);
+    }
+}
+
+static int padFits(int B) {
+    int p = 1;
+    while (p < B) p *= 4;
+    return p;
+}
+
+// Device workspace of the persistent kernel, kept across fits of one engine.
+template <typename Real>
+struct ResWork {
+    int G = 0, BP = 0, B = 0;
+    int64_t N = 0;
+    double bytes = 0.0;
+    // structure (depends on G)
+    int* rowStart = nullptr;
+    int* pidStart = nullptr;
+    int* colOff = nullptr;
+    int* pid = nullptr;       // Cox: unweighted tie pids
+    int* pstart = nullptr;
+    int* pseg = nullptr;
+    // per batch
+    Real* xb = nullptr;
+    double *denom = nullptr, *Dacc = nullptr, *R1 = nullptr, *R2 = nullptr, *nW = nullptr;
+    double *beta = nullptr, *hDelta = nullptr, *var = nullptr, *initBound = nullptr;
+    unsigned char *exclude = nullptr, *fixed = nullptr;
+    double *xjyTot = nullptr, *xjyFold = nullptr, *xjyW = nullptr, *fixedTerm = nullptr;
+    int *held = nullptr, *rowFold = nullptr;
+    int *flag = nullptr, *iters = nullptr, *done = nullptr;
+    double* ll = nullptr;
+    unsigned* epoch = nullptr;
+    double *slots = nullptr, *bcast = nullptr;
+    int* err = nullptr;
+    unsigned long long* steps = nullptr;
+    double* stats = nullptr;
+    Real* kw = nullptr;
+
+    void freeBatch() {
+        dfree(xb); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
+        dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
+        dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold);
+        dfree(flag); dfree(iters); dfree(done); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
+        dfree(err); dfree(steps); dfree(stats); dfree(kw);
+        B = BP = 0;
+    }
+    void freeAll() {
+        freeBatch();
+        dfree(rowStart); dfree(pidStart); dfree(colOff); dfree(pid); dfree(pstart); dfree(pseg);
+        G = 0;
+    }
+};
 
 template <typename Real>
-void EngineT<Real>::fitResident(const ccd_fit_args&, double*, ccd_fit_result&) {
-    throw std::logic_error("resident mode not built");
+void EngineT<Real>::releaseResident() {
+    if (res) {
+        auto* W = static_cast<ResWork<Real>*>(res);
+        W->freeAll();
+        delete W;
+        res = nullptr;
+    }
+}
+
+// Workgroup count of the persistent launch: enough rows per workgroup to amortise
+// the per-coordinate grid exchange, never more workgroups than compute units.
+static int chooseWorkgroups(int64_t K, int device) {
+    int cus = 0;
+    CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (const char* e = std::getenv("CCD_RESIDENT_WG")) {
+        const int v = std::atoi(e);
+        if (v > 0) return std::min(v, std::min(cus, res::MAXG));
+    }
+    int64_t g = (K + 4095) / 4096;
+    if (g < 1) g = 1;
+    if (g > cus) g = cus;
+    if (g > res::MAXG) g = res::MAXG;
+    return (int)g;
+}
+
+template <typename Real, int MODEL, int BP>
+static void launchFindMode(const res::Params<Real>& P, hipStream_t s) {
+    hipLaunchKernelGGL((res::kFindMode<Real, MODEL, BP>), dim3(P.G), dim3(BLOCK), 0, s, P);
+}
+
+template <typename Real, int MODEL>
+static void launchByBP(int BP, const res::Params<Real>& P, hipStream_t s) {
+    switch (BP) {
+    case 1: launchFindMode<Real, MODEL, 1>(P, s); break;
+    case 4: launchFindMode<Real, MODEL, 4>(P, s); break;
+    case 16: launchFindMode<Real, MODEL, 16>(P, s); break;
+    case 64: launchFindMode<Real, MODEL, 64>(P, s); break;
+    case 256: launchFindMode<Real, MODEL, 256>(P, s); break;
+    default: throw std::invalid_argument("unsupported fit padding");
+    }
+}
+
+// Run one persistent launch over fits [0, B) described by the host arrays; fills
+// betas/flags/iters/ll. kwHost: per-row weights of a single fit (or null).
+template <typename Real>
+static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
+                        const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
+                        int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps) {
+    auto& D = *E.D;
+    hipStream_t s = E.s;
+    const int64_t K = D.K;
+    const int J = D.J;
+    const int m = D.model;
+    const bool grouped = (m == M_CLR || m == M_COX);
+    if (!E.res) E.res = new ResWork<Real>();
+    auto& W = *static_cast<ResWork<Real>*>(E.res);
+    const int G = chooseWorkgroups(K, D.device);
+    const int BP = padFits(B);
+    if (BP > 256) throw std::invalid_argument("at most 256 fits per batched launch");
+    // ---- structure for this G -------------------------------------------------
+    if (W.G != G) {
+        W.freeAll();
+        W.G = G;
+        std::vector<int32_t> pid(K), pstart, pseg;
+        int64_t N;
+        if (m == M_COX) {   // unweighted Breslow tie groups (ModelSpecifics.hpp:1809-1868 with no weights)
+            int cur = 0;
+            pid[0] = 0;
+            pseg.push_back(D.strat[0]);
+            for (int64_t k = 1; k < K; ++k) {
+                if (D.strat[k] != D.strat[k - 1]) { ++cur; pseg.push_back(D.strat[k]); }
+                else if (!(D.y[k - 1] == 1.0 && D.time[k - 1] == D.time[k] && D.y[k] == 1.0)) { ++cur; pseg.push_back(D.strat[k]); }
+                pid[k] = cur;
+            }
+            N = cur + 1;
+            pstart.assign(N + 1, 0);
+            for (int64_t k = K - 1; k >= 0; --k) pstart[pid[k]] = (int32_t)k;
+            pstart[N] = (int32_t)K;
+        } else if (m == M_CLR) {
+            N = D.nStrata;
+            pstart = D.stratStart;
+            pseg.resize(N);
+            std::iota(pseg.begin(), pseg.end(), 0);
+        } else {
+            N = K;
+        }
+        W.N = N;
+        // row ranges: even split, moved to the next pid start for grouped models
+        std::vector<int32_t> rs(G + 1), ps(G + 1, 0);
+        for (int w = 0; w <= G; ++w) {
+            int64_t r = (K * w) / G;
+            if (grouped && w > 0 && w < G) {
+                int64_t p = (r < K) ? pid[r] : N;   // first pid starting at or after r
+                if (m == M_CLR) p = (r < K) ? D.strat[r] : N;
+                if (p < N && pstart[p] < r) ++p;
+                r = (p < N) ? pstart[p] : K;
+                ps[w] = (int32_t)p;
+            }
+            rs[w] = (int32_t)r;
+        }
+        for (int w = 1; w <= G; ++w) if (rs[w] < rs[w - 1]) rs[w] = rs[w - 1];
+        if (grouped) {
+            ps[0] = 0;
+            ps[G] = (int32_t)N;
+            for (int w = 1; w <= G; ++w) if (ps[w] < ps[w - 1]) ps[w] = ps[w - 1];
+        }
+        double& Bb = W.bytes;
+        W.rowStart = dalloc<int>(G + 1, Bb);
+        h2d(W.rowStart, rs.data(), G + 1, s);
+        W.pidStart = dalloc<int>(G + 1, Bb);
+        h2d(W.pidStart, ps.data(), G + 1, s);
+        W.colOff = dalloc<int>((size_t)J * (G + 1), Bb);
+        kColOff<<<gridFor((int64_t)J * (G + 1)), BLOCK, 0, s>>>(J, G, D.dFmt, D.dColPtr, D.dRows, W.rowStart, W.colOff);
+        CCD_HIP(hipGetLastError());
+        if (m == M_COX) {
+            W.pid = dalloc<int>(K, Bb);
+            h2d(W.pid, pid.data(), K, s);
+        }
+        if (grouped) {
+            W.pstart = dalloc<int>(N + 1, Bb);
+            h2d(W.pstart, pstart.data(), N + 1, s);
+            W.pseg = dalloc<int>(N + 1, Bb);
+            pseg.push_back(-1);
+            h2d(W.pseg, pseg.data(), N + 1, s);
+        }
+        CCD_HIP(hipStreamSynchronize(s));
+    }
+    const int64_t N = W.N;
+    // ---- per-batch buffers ------------------------------------------------------
+    if (W.B != B || W.BP != BP) {
+        W.freeBatch();
+        W.B = B;
+        W.BP = BP;
+        double& Bb = W.bytes;
+        W.xb = dalloc<Real>((size_t)K * BP, Bb);
+        if (grouped) {
+            W.denom = dalloc<double>((size_t)N * BP, Bb);
+            W.Dacc = dalloc<double>((size_t)N * BP, Bb);
+            W.R1 = dalloc<double>((size_t)N * BP, Bb);
+            W.R2 = dalloc<double>((size_t)N * BP, Bb);
+            W.nW = dalloc<double>((size_t)N * BP, Bb);
+        }
+        W.beta = dalloc<double>((size_t)B * J, Bb);
+        W.hDelta = dalloc<double>((size_t)B * J, Bb);
+        W.var = dalloc<double>(B, Bb);
+        W.initBound = dalloc<double>(B, Bb);
+        W.exclude = dalloc<unsigned char>(J, Bb);
+        W.fixed = dalloc<unsigned char>(J, Bb);
+        W.xjyTot = dalloc<double>(J, Bb);
+        W.xjyFold = dalloc<double>((size_t)J * 32, Bb);
+        W.xjyW = dalloc<double>(J, Bb);
+        W.fixedTerm = dalloc<double>(B, Bb);
+        W.held = dalloc<int>(B, Bb);
+        W.rowFold = dalloc<int>(K, Bb);
+        W.flag = dalloc<int>(B, Bb);
+        W.iters = dalloc<int>(B, Bb);
+        W.done = dalloc<int>(B, Bb);
+        W.ll = dalloc<double>(B, Bb);
+        W.epoch = dalloc<unsigned>(G, Bb);
+        W.slots = dalloc<double>((size_t)2 * G * BP * res::SLOTW, Bb);
+        W.bcast = dalloc<double>((size_t)2 * BP * 4, Bb);
+        W.err = dalloc<int>(1, Bb);
+        W.steps = dalloc<unsigned long long>(1, Bb);
+        W.stats = dalloc<double>(4, Bb);
+        W.kw = dalloc<Real>(K, Bb);
+    }
+    // ---- inputs ---------------------------------------------------------------
+    std::vector<unsigned char> excl(J, 0), fix(J, 0);
+    for (int j = 0; j < J; ++j) {
+        if (a.exclude && a.exclude[j]) excl[j] = 1;
+        if (a.fixed && a.fixed[j]) fix[j] = 1;
+    }
+    if (D.hasOffset) fix[0] = 1;
+    std::vector<double> hd((size_t)B * J);
+    for (int f = 0; f < B; ++f) std::fill(hd.begin() + (size_t)f * J, hd.begin() + (size_t)(f + 1) * J, initBound[f]);
+    h2d(W.beta, betas, (size_t)B * J, s);
+    h2d(W.hDelta, hd.data(), (size_t)B * J, s);
+    h2d(W.var, var, B, s);
+    h2d(W.initBound, initBound, B, s);
+    h2d(W.exclude, excl.data(), J, s);
+    h2d(W.fixed, fix.data(), J, s);
+    std::vector<int32_t> hh(B, -1);
+    if (held) std::copy(held, held + B, hh.begin());
+    h2d(W.held, hh.data(), B, s);
+    if (rowFold) h2d(W.rowFold, rowFold, K, s);
+    if (kwHost) h2dConvert(W.kw, kwHost, K, s);
+    // XjY: total (and per fold), or weighted for a single weighted fit
+    {
+        const int grid = std::min(J, 4096);
+        if (kwHost) {
+            kXjY<Real, true><<<grid, BLOCK, 0, s>>>(J, K, D.dFmt, D.dColPtr, D.dDensePtr, D.dRows, D.dVals, D.dDense,
+                                                   D.dY, W.kw, W.xjyW);
+        }
+        kXjYFold<Real><<<grid, BLOCK, 0, s>>>(J, K, F, D.dFmt, D.dColPtr, D.dDensePtr, D.dRows, D.dVals, D.dDense,
+                                              D.dY, rowFold ? W.rowFold : nullptr, W.xjyTot,
+                                              rowFold ? W.xjyFold : nullptr);
+        CCD_HIP(hipGetLastError());
+    }
+    // Poisson fixed term per fit: sum_k w_fk * (-log y_k!)
+    std::vector<double> ft(B, 0.0);
+    if (m == M_POISSON) {
+        std::vector<double> byFold(F > 0 ? F : 1, 0.0);
+        double total = 0.0, totalW = 0.0;
+        for (int64_t k = 0; k < K; ++k) {
+            double c = 0.0;
+            for (int i = 2; i <= (int)D.y[k]; ++i) c -= std::log((double)i);
+            total += c;
+            if (rowFold) byFold[rowFold[k]] += c;
+            if (kwHost) totalW += c * kwHost[k];
+        }
+        for (int f = 0; f < B; ++f)
+            ft[f] = kwHost ? totalW : total - ((rowFold && hh[f] >= 0) ? byFold[hh[f]] : 0.0);
+    }
+    h2d(W.fixedTerm, ft.data(), B, s);
+    // xb = X beta for every fit (column order, as CyclicCoordinateDescent::computeXBeta)
+    CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)K * BP * sizeof(Real), s));
+    {
+        std::vector<double> colBeta(B);
+        double* dColBeta = nullptr;
+        double tmpBytes = 0.0;
+        dColBeta = dalloc<double>((size_t)J * B, tmpBytes);
+        std::vector<double> all((size_t)J * B);
+        for (int j = 0; j < J; ++j)
+            for (int f = 0; f < B; ++f) all[(size_t)j * B + f] = betas[(size_t)f * J + j];
+        h2d(dColBeta, all.data(), (size_t)J * B, s);
+        for (int j = 0; j < J; ++j) {
+            bool any = false;
+            for (int f = 0; f < B; ++f) any |= all[(size_t)j * B + f] != 0.0;
+            if (!any) continue;
+            const Col<Real> c = E.col(j);
+            if (c.n == 0) continue;
+            CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, dColBeta + (size_t)j * B, B, BP, W.xb)));
+        }
+        CCD_HIP(hipGetLastError());
+        CCD_HIP(hipStreamSynchronize(s));
+        dfree(dColBeta);
+    }
+    CCD_HIP(hipMemsetAsync(W.epoch, 0, G * sizeof(unsigned), s));
+    CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
+    CCD_HIP(hipMemsetAsync(W.steps, 0, sizeof(unsigned long long), s));
+    CCD_HIP(hipMemsetAsync(W.done, 0, B * sizeof(int), s));
+    CCD_HIP(hipMemsetAsync(W.flag, 0, B * sizeof(int), s));
+    CCD_HIP(hipMemsetAsync(W.iters, 0, B * sizeof(int), s));
+    // ---- parameters -----------------------------------------------------------
+    res::Params<Real> P{};
+    P.K = K; P.J = J; P.G = G; P.B = B; P.N = grouped ? N : K;
+    P.fmt = D.dFmt; P.colPtr = D.dColPtr; P.densePtr = D.dDensePtr; P.rows = D.dRows; P.vals = D.dVals;
+    P.dense = D.dDense; P.y = D.dY;
+    P.seg = D.dStrat;
+    P.pid = (m == M_COX) ? W.pid : D.dStrat;
+    P.pstart = W.pstart; P.pseg = W.pseg;
+    P.rowStart = W.rowStart; P.pidStart = W.pidStart; P.colOff = W.colOff;
+    P.kw = kwHost ? W.kw : nullptr;
+    P.rowFold = rowFold ? W.rowFold : nullptr;
+    P.held = W.held;
+    P.xb = W.xb; P.denom = W.denom; P.Dacc = W.Dacc; P.R1 = W.R1; P.R2 = W.R2; P.nW = W.nW;
+    P.beta = W.beta; P.hDelta = W.hDelta; P.var = W.var; P.initBound = W.initBound;
+    P.exclude = W.exclude; P.fixed = W.fixed;
+    P.xjyTot = W.xjyTot; P.xjyFold = rowFold ? W.xjyFold : nullptr; P.F = F; P.xjyW = kwHost ? W.xjyW : nullptr;
+    P.fixedTermFit = W.fixedTerm;
+    P.priorType = a.prior_type; P.convType = a.convergence_type; P.tol = a.tolerance; P.maxIt = a.max_iterations;
+    P.flag = W.flag; P.done = W.done; P.iters = W.iters; P.ll = W.ll;
+    P.epoch = W.epoch; P.slots = W.slots; P.bcast = W.bcast; P.err = W.err;
+    int rateKHz = 0;
+    CCD_HIP(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, D.device));
+    if (rateKHz <= 0) rateKHz = 100000;
+    P.deadlineTicks = (unsigned long long)rateKHz * 1000ull * 20ull;   // 20 s for one grid exchange
+    P.allRead = (B <= 2) ? 1 : 0;
+    P.steps = W.steps;
+    // ---- launch -----------------------------------------------------------------
+    CCD_HIP(hipEventRecord(E.ev0, s));
+    switch (m) {
+    case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, P, s); break;
+    case M_POISSON: launchByBP<Real, M_POISSON>(BP, P, s); break;
+    case M_CLR: launchByBP<Real, M_CLR>(BP, P, s); break;
+    default: launchByBP<Real, M_COX>(BP, P, s); break;
+    }
+    CCD_HIP(hipGetLastError());
+    CCD_HIP(hipEventRecord(E.ev1, s));
+    CCD_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    CCD_HIP(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+    deviceMs = ms;
+    int errFlag = 0;
+    unsigned long long st = 0;
+    CCD_HIP(hipMemcpy(&errFlag, W.err, sizeof(int), hipMemcpyDeviceToHost));
+    CCD_HIP(hipMemcpy(&st, W.steps, sizeof(st), hipMemcpyDeviceToHost));
+    if (errFlag) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
+    steps = (double)st;
+    CCD_HIP(hipMemcpy(betas, W.beta, (size_t)B * J * sizeof(double), hipMemcpyDeviceToHost));
+    CCD_HIP(hipMemcpy(flags, W.flag, B * sizeof(int), hipMemcpyDeviceToHost));
+    CCD_HIP(hipMemcpy(iters, W.iters, B * sizeof(int), hipMemcpyDeviceToHost));
+    CCD_HIP(hipMemcpy(lls, W.ll, B * sizeof(double), hipMemcpyDeviceToHost));
+    // compulsory bytes of the column passes actually run (per coordinate step)
+    double colBytes = 0.0;
+    for (int j = 0; j < J; ++j) {
+        if (fix[j]) continue;
+        const int f = D.fmt[j];
+        const double n = (double)((f == F_INDICATOR || f == F_SPARSE) ? D.colPtr[j + 1] - D.colPtr[j] : K);
+        double b = n * B * sizeof(Real);
+        if (f == F_INDICATOR || f == F_SPARSE) b += n * 4.0;
+        if (f == F_SPARSE || f == F_DENSE) b += n * sizeof(Real);
+        if (rowFold || kwHost) b += n * 4.0;
+        if (grouped) b += n * (8.0 + 16.0 * B);
+        colBytes += b;
+    }
+    int maxIt = 0;
+    for (int f = 0; f < B; ++f) maxIt = std::max(maxIt, iters[f]);
+    algoBytes = colBytes * maxIt;
 }
 
 template <typename Real>
-void EngineT<Real>::fitBatch(const ccd_fit_args&, const ccd_batch_args&, double*, ccd_fit_result*) {
-    throw std::logic_error("batched mode not built");
+void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) {
+    CCD_HIP(hipSetDevice(D->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int J = D->J;
+    std::vector<double> b(beta, beta + J);
+    if (D->hasOffset) b[0] = 1.0;
+    double initBound = a.initial_bound;
+    int count = 0, flag = 0, it = 0;
+    double ll = 0.0;
+    double var = a.variance;
+    for (;;) {
+        double ms = 0, bytes = 0, steps = 0;
+        runResident<Real>(*this, a, 1, &var, nullptr, nullptr, 0, a.weights, &initBound, b.data(), &flag, &it, &ll,
+                          ms, bytes, steps);
+        ++count;
+        if (timing) {
+            counters.deviceMs += ms;
+            counters.launches += 1;
+            counters.bytes += bytes;
+            counters.coordinates += steps;
+        }
+        if (flag == CCD_ILLCONDITIONED && count < a.max_bound_count) {   // update() retry (:599-614)
+            initBound /= 10.0;
+            for (int j = D->hasOffset ? 1 : 0; j < J; ++j) b[j] = 0.0;
+        } else {
+            break;
+        }
+    }
+    std::copy(b.begin(), b.end(), beta);
+    double lp = 0.0;
+    for (int j = 0; j < J; ++j)
+        lp += priorLogDensity((a.exclude && a.exclude[j]) ? CCD_PRIOR_NONE : a.prior_type, a.variance, b[j]);
+    r.return_flag = flag;
+    r.iterations = it;
+    r.find_mode_calls = count;
+    r.log_likelihood = ll;
+    r.log_prior = lp;
+    r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename Real>
+void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas, ccd_fit_result* r) {
+    CCD_HIP(hipSetDevice(D->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int J = D->J;
+    const int n = bargs.n_fits;
+    int F = 0;
+    if (bargs.row_fold) {
+        for (int64_t k = 0; k < D->K; ++k) {
+            const int q = bargs.row_fold[k];
+            if (q < 0 || q >= 32) throw std::invalid_argument("row_fold values must be in [0, 32)");
+            F = std::max(F, q + 1);
+        }
+    }
+    if (bargs.held_out_fold && !bargs.row_fold)
+        for (int f = 0; f < n; ++f)
+            if (bargs.held_out_fold[f] >= 0) throw std::invalid_argument("held_out_fold needs row_fold");
+    for (int f = 0; f < n; ++f) {
+        if (a.prior_type != CCD_PRIOR_NONE && !(bargs.variances[f] > 0))
+            throw std::invalid_argument("variances must be > 0");
+        if (D->hasOffset) betas[(size_t)f * J] = 1.0;
+    }
+    std::vector<int> count(n, 0), flags(n, 0), its(n, 0);
+    std::vector<double> lls(n, 0.0), bound(n, a.initial_bound);
+    std::vector<int> pending(n);
+    std::iota(pending.begin(), pending.end(), 0);
+    while (!pending.empty()) {
+        // chunks of at most 256 fits per launch
+        std::vector<int> retry;
+        for (size_t c0 = 0; c0 < pending.size(); c0 += 256) {
+            const int B = (int)std::min<size_t>(256, pending.size() - c0);
+            std::vector<double> v(B), bd(B), bb((size_t)B * J), l(B);
+            std::vector<int32_t> h(B);
+            std::vector<int> fl(B), it(B);
+            for (int i = 0; i < B; ++i) {
+                const int f = pending[c0 + i];
+                v[i] = bargs.variances[f];
+                h[i] = bargs.held_out_fold ? bargs.held_out_fold[f] : -1;
+                bd[i] = bound[f];
+                std::copy(betas + (size_t)f * J, betas + (size_t)(f + 1) * J, bb.begin() + (size_t)i * J);
+            }
+            double ms = 0, bytes = 0, steps = 0;
+            runResident<Real>(*this, a, B, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(), bb.data(),
+                              fl.data(), it.data(), l.data(), ms, bytes, steps);
+            if (timing) {
+                counters.deviceMs += ms;
+                counters.launches += 1;
+                counters.bytes += bytes;
+                counters.coordinates += steps;
+            }
+            for (int i = 0; i < B; ++i) {
+                const int f = pending[c0 + i];
+                std::copy(bb.begin() + (size_t)i * J, bb.begin() + (size_t)(i + 1) * J, betas + (size_t)f * J);
+                flags[f] = fl[i];
+                its[f] = it[i];
+                lls[f] = l[i];
+                count[f] += 1;
+                if (fl[i] == CCD_ILLCONDITIONED && count[f] < a.max_bound_count) {
+                    bound[f] /= 10.0;
+                    for (int j = D->hasOffset ? 1 : 0; j < J; ++j) betas[(size_t)f * J + j] = 0.0;
+                    retry.push_back(f);
+                }
+            }
+        }
+        pending.swap(retry);
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int f = 0; f < n; ++f) {
+        double lp = 0.0;
+        for (int j = 0; j < J; ++j)
+            lp += priorLogDensity((a.exclude && a.exclude[j]) ? CCD_PRIOR_NONE : a.prior_type, bargs.variances[f],
+                                  betas[(size_t)f * J + j]);
+        r[f].return_flag = flags[f];
+        r[f].iterations = its[f];
+        r[f].find_mode_calls = count[f];
+        r[f].log_likelihood = lls[f];
+        r[f].log_prior = lp;
+        r[f].seconds = secs;
+    }
 }
 
 }  // namespace ccd

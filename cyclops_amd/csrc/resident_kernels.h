@@ -1,0 +1,903 @@
+// resident_kernels.h -- persistent CCD kernels: a whole findMode (or B fits of the
+// same data in lockstep) inside one launch.
+//
+// Work split: workgroup w owns the rows [rowStart[w], rowStart[w+1]) (aligned to
+// stratum / pid boundaries for CLR and Cox) and so owns every per-row and per-pid
+// value of those rows for every fit. Per coordinate j:
+//   1. each workgroup reduces the entries of column j that fall in its rows;
+//   2. the per-workgroup partials meet in a grid all-reduce (epoch-tagged slots,
+//      no atomics, fixed reduction order -> every workgroup sees identical g, h);
+//   3. the FP64 prior step and trust-region bound give delta (identical in every
+//      workgroup, or computed once by the fit's owner and broadcast);
+//   4. each workgroup applies delta to its own rows; Cox then re-forms the risk-set
+//      sums with two more grid scans.
+// State is fit-minor: xb[row * BP + fit], pid arrays [pid * BP + fit].
+#pragma once
+
+#include "kernels.h"
+
+namespace ccd {
+namespace res {
+
+// Ordered reduction state of a column pass (see kGradGrouped for the algebra):
+// g, h sums; l quadratic term; m pre-reset R2 sum; t trailing segment sum; r reset.
+struct Mon {
+    double g, h, l, m, t, r;
+};
+__device__ __forceinline__ Mon monZero() { return Mon{0, 0, 0, 0, 0, 0}; }
+__device__ __forceinline__ Mon monCombine(const Mon& a, const Mon& b) {
+    Mon c;
+    c.g = a.g + b.g;
+    c.h = a.h + b.h;
+    c.l = a.l + b.l + 2.0 * a.t * b.m;
+    c.m = (a.r != 0.0) ? a.m : a.m + b.m;
+    c.t = (b.r != 0.0) ? b.t : a.t + b.t;
+    c.r = (a.r != 0.0 || b.r != 0.0) ? 1.0 : 0.0;
+    return c;
+}
+
+template <typename Real>
+struct Params {
+    // data
+    int64_t K;
+    int J;
+    int G;            // workgroups in the launch
+    int B;            // fits
+    int64_t N;        // pids (CLR: strata; Cox: tie groups); K for LR/PR
+    const int* fmt;
+    const int64_t* colPtr;
+    const int64_t* densePtr;
+    const int* rows;
+    const Real* vals;
+    const Real* dense;
+    const Real* y;
+    const int* seg;       // stratum per row (CLR/Cox)
+    const int* pid;       // pid per row (CLR: stratum, Cox: tie group)
+    const int* pstart;    // [N+1] first row of each pid
+    const int* pseg;      // [N] stratum of each pid
+    const int* rowStart;  // [G+1]
+    const int* pidStart;  // [G+1]
+    const int* colOff;    // [J*(G+1)] entry offset of workgroup w inside column j (INDICATOR/SPARSE)
+    // weights
+    const Real* kw;       // [K] (single fit) or null
+    const int* rowFold;   // [K] or null
+    const int* held;      // [B]
+    // fit state
+    Real* xb;             // [K*BP]
+    double* denom;        // [N*BP] (CLR/Cox)
+    double* Dacc;         // [N*BP] (Cox)
+    double* R1;           // [N*BP]
+    double* R2;           // [N*BP]
+    double* nW;           // [N*BP]
+    double* beta;         // [B*J]
+    double* hDelta;       // [B*J]
+    const double* var;    // [B]
+    const double* initBound;   // [B]
+    const unsigned char* exclude;   // [J]
+    const unsigned char* fixed;     // [J]
+    const double* xjyTot;   // [J]
+    const double* xjyFold;  // [J*F] or null
+    int F;
+    const double* xjyW;     // [J] (weighted single fit) or null
+    const double* fixedTermFit;   // [B] Poisson fixed term per fit
+    int priorType;
+    int convType;
+    double tol;
+    int maxIt;
+    // results
+    int* flag;            // [B]
+    int* done;            // [B] owner-mode convergence state
+    int* iters;           // [B]
+    double* ll;           // [B]
+    // sync / scratch
+    unsigned* epoch;      // [G]
+    double* slots;        // [2][G][BP][6]
+    double* bcast;        // [2][BP][4]
+    int* err;
+    unsigned long long deadlineTicks;
+    int allRead;
+    // counters
+    unsigned long long* steps;
+};
+
+constexpr int SLOTW = 6;
+
+__device__ __forceinline__ unsigned long long now() { return wall_clock64(); }
+
+// Store-release of this workgroup's epoch after every thread's writes are fenced.
+__device__ __forceinline__ void publish(unsigned* epoch, unsigned seq) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&epoch[blockIdx.x], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until every workgroup has published `seq`. Returns false on timeout or a
+// raised error flag (every waiter then leaves the kernel).
+__device__ __forceinline__ bool waitAll(const unsigned* epoch, unsigned seq, int G, int* err,
+                                        unsigned long long ticks) {
+    const unsigned long long deadline = now() + ticks;
+    for (;;) {
+        int ok = 1;
+        for (int w = threadIdx.x; w < G; w += BLOCK)
+            ok &= (__hip_atomic_load(&epoch[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= seq) ? 1 : 0;
+        if (__syncthreads_and(ok)) break;
+        int bad = 0;
+        if (threadIdx.x == 0) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) bad = 1;
+            else if (now() > deadline) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bad = 1;
+            }
+        }
+        if (__syncthreads_or(bad)) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __threadfence();
+    return true;
+}
+
+template <typename Real>
+__device__ __forceinline__ Real weightOf(const Params<Real>& P, int64_t k, int f) {
+    if (P.rowFold) return (P.rowFold[k] == P.held[f]) ? Real(0) : Real(1);
+    if (P.kw) return P.kw[k];
+    return Real(1);
+}
+
+}  // namespace res
+}  // namespace ccd
+
+namespace ccd {
+namespace res {
+
+constexpr int MAXG = 512;
+
+template <int BP>
+struct Geo {
+    static constexpr int SUB = BP < 64 ? BP : 64;   // lanes per entry
+    static constexpr int NSG = BLOCK / SUB;         // sub-groups per workgroup
+    static constexpr int FPL = BP / SUB;            // fits per lane
+};
+
+template <typename Real, int FMT>
+__device__ __forceinline__ int64_t rowAt(const Params<Real>& P, int64_t e) {
+    if constexpr (FMT == F_INDICATOR || FMT == F_SPARSE) return P.rows[e];
+    else return e;
+}
+template <typename Real, int FMT>
+__device__ __forceinline__ Real xAt(const Params<Real>& P, int j, int64_t e) {
+    if constexpr (FMT == F_SPARSE) return P.vals[e];
+    else if constexpr (FMT == F_DENSE) return P.dense[P.densePtr[j] + e];
+    else return Real(1);
+}
+
+// Entry range of workgroup w in column j.
+template <typename Real>
+__device__ __forceinline__ void colRange(const Params<Real>& P, int j, int w, int64_t& a, int64_t& b, int64_t& c0) {
+    const int f = P.fmt[j];
+    if (f == F_INDICATOR || f == F_SPARSE) {
+        c0 = P.colPtr[j];
+        const int* off = P.colOff + (size_t)j * (P.G + 1);
+        a = c0 + off[w];
+        b = c0 + off[w + 1];
+    } else {
+        c0 = 0;
+        a = P.rowStart[w];
+        b = P.rowStart[w + 1];
+    }
+}
+
+// Sub-group chunk of [a, b); for grouped models the boundaries move forward to the
+// next change of pid so that a run of equal pid is never split.
+template <typename Real, int FMT, bool GROUPED>
+__device__ __forceinline__ void subChunk(const Params<Real>& P, int64_t a, int64_t b, int sg, int nsg,
+                                         int64_t& n0, int64_t& n1) {
+    const int64_t c = (b - a + nsg - 1) / nsg;
+    n0 = a + (int64_t)sg * c;
+    n1 = n0 + c;
+    if (n0 > b) n0 = b;
+    if (n1 > b) n1 = b;
+    if constexpr (GROUPED) {
+        while (n0 > a && n0 < b && P.pid[rowAt<Real, FMT>(P, n0)] == P.pid[rowAt<Real, FMT>(P, n0 - 1)]) ++n0;
+        while (n1 > a && n1 < b && P.pid[rowAt<Real, FMT>(P, n1)] == P.pid[rowAt<Real, FMT>(P, n1 - 1)]) ++n1;
+    }
+}
+
+template <typename Real, int MODEL, int BP, int FMT>
+__device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t a, int64_t b, int64_t c0,
+                                         Mon (&acc)[Geo<BP>::FPL]) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    int64_t n0, n1;
+    subChunk<Real, FMT, GROUPED>(P, a, b, sg, GE::NSG, n0, n1);
+    int prevSeg = 0;
+    bool hasPrev = false;
+    if constexpr (GROUPED) {
+        if (n0 < n1 && n0 > c0) {
+            prevSeg = P.seg[rowAt<Real, FMT>(P, n0 - 1)];
+            hasPrev = true;
+        }
+    }
+    for (int64_t e = n0; e < n1; ++e) {
+        const int64_t k = rowAt<Real, FMT>(P, e);
+        const Real x = xAt<Real, FMT>(P, j, e);
+        bool reset = false;
+        int p = 0;
+        if constexpr (GROUPED) {
+            const int sgm = P.seg[k];
+            reset = hasPrev && sgm != prevSeg;
+            prevSeg = sgm;
+            hasPrev = true;
+            p = P.pid[k];
+        }
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            const int f = sl + i * GE::SUB;
+            if (f >= P.B) continue;
+            const Real xv = P.xb[k * BP + f];
+            const Real ee = dexp(xv);
+            if constexpr (MODEL == M_LOGISTIC) {
+                const Real wt = weightOf(P, k, f);
+                const Real pr = ee / (Real(1) + ee);
+                acc[i].g += (double)(wt * pr * x);
+                acc[i].h += (double)(wt * x * x * pr * (Real(1) - pr));
+            } else if constexpr (MODEL == M_POISSON) {
+                const Real wt = weightOf(P, k, f);
+                acc[i].g += (double)(wt * ee * x);
+                acc[i].h += (double)(wt * ee * x * x);
+            } else {
+                Real uu = ee * x;
+                if constexpr (MODEL == M_COX) uu *= weightOf(P, k, f);
+                const double u = (double)uu;
+                const double r1 = P.R1[(size_t)p * BP + f], r2 = P.R2[(size_t)p * BP + f];
+                if (reset) {
+                    acc[i].t = 0.0;
+                    acc[i].r = 1.0;
+                }
+                const double s = acc[i].t + u;
+                acc[i].l += r2 * u * (2.0 * s - u);
+                if (acc[i].r == 0.0) acc[i].m += r2 * u;
+                acc[i].t = s;
+                acc[i].g += u * r1;
+                acc[i].h += u * (double)x * r1;
+            }
+        }
+    }
+}
+
+template <typename Real, int MODEL, int BP, int FMT>
+__device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t a, int64_t b,
+                                           const double* sDelta) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    int64_t n0, n1;
+    subChunk<Real, FMT, GROUPED>(P, a, b, sg, GE::NSG, n0, n1);
+    double acc[GE::FPL];
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
+    for (int64_t e = n0; e < n1; ++e) {
+        const int64_t k = rowAt<Real, FMT>(P, e);
+        const Real x = xAt<Real, FMT>(P, j, e);
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            const int f = sl + i * GE::SUB;
+            if (f >= P.B) continue;
+            const double d = sDelta[f];
+            if (d == 0.0) continue;
+            Real* px = P.xb + k * BP + f;
+            const Real old = *px;
+            const Real nv = old + (Real)d * x;
+            *px = nv;
+            if constexpr (GROUPED) {
+                Real diff = dexp(nv) - dexp(old);
+                if constexpr (MODEL == M_COX) diff *= weightOf(P, k, f);
+                acc[i] += (double)diff;
+            }
+        }
+        if constexpr (GROUPED) {
+            const int p = P.pid[k];
+            const bool runEnd = (e + 1 >= n1) || P.pid[rowAt<Real, FMT>(P, e + 1)] != p;
+            if (runEnd) {
+#pragma unroll
+                for (int i = 0; i < GE::FPL; ++i) {
+                    const int f = sl + i * GE::SUB;
+                    if (f >= P.B || sDelta[f] == 0.0) continue;
+                    const size_t q = (size_t)p * BP + f;
+                    const double dn = P.denom[q] + acc[i];
+                    P.denom[q] = dn;
+                    if constexpr (MODEL == M_CLR) {
+                        const double nw = P.nW[q];
+                        const double r = nw == 0.0 ? 0.0 : nw / dn;
+                        P.R1[q] = r;
+                        P.R2[q] = nw == 0.0 ? 0.0 : r / dn;
+                    }
+                    acc[i] = 0.0;
+                }
+            }
+        }
+    }
+}
+
+// Ordered tree combine of NSG sub-group Mons per fit in LDS; result in sMon[f].
+template <int BP>
+__device__ __forceinline__ void combineSubgroups(Mon* sMon) {
+    using GE = Geo<BP>;
+    for (int s = 1; s < GE::NSG; s <<= 1) {
+        const int pairs = GE::NSG / (2 * s);
+        for (int idx = threadIdx.x; idx < pairs * BP; idx += BLOCK) {
+            const int q = idx / BP, f = idx % BP;
+            const int a = 2 * s * q, b = a + s;
+            sMon[a * BP + f] = monCombine(sMon[a * BP + f], sMon[b * BP + f]);
+        }
+        __syncthreads();
+    }
+}
+
+// Ordered combine of the G workgroup partials of fit f (slots) -> thread 0 gets it.
+__device__ __forceinline__ Mon reduceSlots(const double* slot, int G, int BP, int f, Mon* sRed) {
+    int P2 = 1;
+    while (P2 < G) P2 <<= 1;
+    for (int w = threadIdx.x; w < P2; w += BLOCK) {
+        if (w < G) {
+            const double* s = slot + ((size_t)w * BP + f) * SLOTW;
+            sRed[w] = Mon{s[0], s[1], s[2], s[3], s[4], s[5]};
+        } else {
+            sRed[w] = monZero();
+        }
+    }
+    __syncthreads();
+    for (int s = 1; s < P2; s <<= 1) {
+        for (int q = threadIdx.x; q < P2 / (2 * s); q += BLOCK) {
+            const int a = 2 * s * q;
+            sRed[a] = monCombine(sRed[a], sRed[a + s]);
+        }
+        __syncthreads();
+    }
+    const Mon r = sRed[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double priorDeltaDev(int type, double variance, double g, double h, double beta) {
+    if (type == 0) return -(g / h);
+    if (type == 2) return -(g + beta / variance) / (h + 1.0 / variance);
+    const double lambda = sqrt(2.0 / variance);
+    const double negU = -(g - lambda) / h;
+    const double posU = -(g + lambda) / h;
+    const int sb = beta == 0 ? 0 : (beta < 0 ? -1 : 1);
+    double delta;
+    if (sb == 0) {
+        if (negU < 0) delta = negU;
+        else if (posU > 0) delta = posU;
+        else delta = 0;
+    } else {
+        delta = (sb < 0) ? negU : posU;
+        const double nb = beta + delta;
+        const int sn = nb == 0 ? 0 : (nb < 0 ? -1 : 1);
+        if (sn != sb) delta = -beta;
+    }
+    return delta;
+}
+
+__device__ __forceinline__ double logDensityDev(int type, double variance, double x) {
+    if (type == 0) return 0.0;
+    if (type == 1) {
+        const double l = sqrt(2.0 / variance);
+        return log(0.5 * l) - l * fabs(x);
+    }
+    return -0.5 * log(2.0 * M_PI * variance) - 0.5 * x * x / variance;
+}
+
+}  // namespace res
+}  // namespace ccd
+
+namespace ccd {
+namespace res {
+
+// Ordered tree combine of n items (loaded by ld(i)) in LDS; every thread gets the result.
+template <class T, class Load, class Comb>
+__device__ __forceinline__ T treeReduce(int n, Load ld, Comb cb, T* sBuf, T ident) {
+    int P2 = 1;
+    while (P2 < n) P2 <<= 1;
+    for (int i = threadIdx.x; i < P2; i += BLOCK) sBuf[i] = (i < n) ? ld(i) : ident;
+    __syncthreads();
+    for (int s = 1; s < P2; s <<= 1) {
+        for (int q = threadIdx.x; q < P2 / (2 * s); q += BLOCK) {
+            const int a = 2 * s * q;
+            sBuf[a] = cb(sBuf[a], sBuf[a + s]);
+        }
+        __syncthreads();
+    }
+    const T r = sBuf[0];
+    __syncthreads();
+    return r;
+}
+
+// Shared scratch of the persistent kernel.
+template <int BP>
+struct Shared {
+    Mon mon[Geo<BP>::NSG * BP];
+    union {
+        Mon red[MAXG];
+        Seg2 seg[MAXG];
+    } u;
+    Seg2 scan[Geo<BP>::NSG * BP];
+    double out[BP * 4];
+    double beta[BP];
+    double hd[BP];
+    double carry[BP];
+    double carry2[BP];
+    // per-fit control (all-read mode: every workgroup keeps an identical copy)
+    double last[BP];
+    int iter[BP];
+    int done[BP];
+    int flag[BP];
+};
+
+// Publish slots, wait, reduce per fit (all-read: every workgroup reduces every fit;
+// owner mode: workgroup f % G reduces fit f and broadcasts 4 doubles). On return
+// s.out[4f..4f+3] holds fn's output for every fit in every workgroup.
+template <typename Real, int BP, class Fn>
+__device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
+                                           Shared<BP>& s, Fn fn) {
+    publish(P.epoch, ++seq);
+    if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+    const double* slot = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
+    auto comb = [](const Mon& a, const Mon& b) { return monCombine(a, b); };
+    if (P.allRead) {
+        for (int f = 0; f < P.B; ++f) {
+            const Mon m = treeReduce<Mon>(P.G, [&](int w) {
+                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+                return Mon{q[0], q[1], q[2], q[3], q[4], q[5]};
+            }, comb, s.u.red, monZero());
+            if (threadIdx.x == 0) fn(f, m, s.out + 4 * f);
+        }
+        __syncthreads();
+    } else {
+        double* bc = P.bcast + (size_t)((seq + 1) & 1) * BP * 4;
+        for (int f = blockIdx.x; f < P.B; f += P.G) {
+            const Mon m = treeReduce<Mon>(P.G, [&](int w) {
+                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+                return Mon{q[0], q[1], q[2], q[3], q[4], q[5]};
+            }, comb, s.u.red, monZero());
+            if (threadIdx.x == 0) {
+                double o[4] = {0, 0, 0, 0};
+                fn(f, m, o);
+                for (int i = 0; i < 4; ++i) bc[f * 4 + i] = o[i];
+            }
+        }
+        publish(P.epoch, ++seq);
+        if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+        for (int i = threadIdx.x; i < P.B * 4; i += BLOCK) s.out[i] = bc[i];
+        __syncthreads();
+    }
+    return true;
+}
+
+// Write this workgroup's per-fit Mons (s.mon[f]) to the slots of the next epoch.
+template <typename Real, int BP>
+__device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, const Shared<BP>& s) {
+    double* slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
+    for (int f = threadIdx.x; f < BP; f += BLOCK) {
+        double* q = slot + ((size_t)blockIdx.x * BP + f) * SLOTW;
+        const Mon& m = s.mon[f];
+        q[0] = m.g; q[1] = m.h; q[2] = m.l; q[3] = m.m; q[4] = m.t; q[5] = m.r;
+    }
+}
+
+// Breslow risk sets of this workgroup's pids, both scans, with the cross-workgroup carries.
+template <typename Real, int BP>
+__device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
+                                          Shared<BP>& s) {
+    using GE = Geo<BP>;
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    const int w = blockIdx.x;
+    const int64_t q0 = P.pidStart[w], q1 = P.pidStart[w + 1];
+    const int64_t c = (q1 - q0 + GE::NSG - 1) / GE::NSG;
+    int64_t p0 = q0 + sg * c, p1 = p0 + c;
+    if (p0 > q1) p0 = q1;
+    if (p1 > q1) p1 = q1;
+    const int64_t N = P.N;
+    auto segc = [](const Seg2& a, const Seg2& b) { return segCombine(a, b); };
+    const Seg2 z{0.0, 0.0, 0, 0};
+    // ---- forward: D = segmented prefix of denom
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        const int f = sl + i * GE::SUB;
+        Seg2 agg = z;
+        for (int64_t p = p0; p < p1; ++p)
+            agg = segCombine(agg, Seg2{P.denom[p * BP + f], 0.0, (p > 0 && P.pseg[p] != P.pseg[p - 1]) ? 1 : 0, 0});
+        s.scan[sg * BP + f] = agg;
+    }
+    __syncthreads();
+    double* slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
+    for (int f = threadIdx.x; f < BP; f += BLOCK) {
+        Seg2 run = z;
+        for (int q = 0; q < GE::NSG; ++q) {
+            const Seg2 v = s.scan[q * BP + f];
+            s.scan[q * BP + f] = run;
+            run = segCombine(run, v);
+        }
+        double* o = slot + ((size_t)w * BP + f) * SLOTW;
+        o[0] = run.a; o[1] = run.b; o[2] = run.f;
+    }
+    publish(P.epoch, ++seq);
+    if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+    {
+        const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
+        for (int f = 0; f < BP; ++f) {
+            const Seg2 cr = treeReduce<Seg2>(w, [&](int q) {
+                const double* o = sl2 + ((size_t)q * BP + f) * SLOTW;
+                return Seg2{o[0], o[1], (int)o[2], 0};
+            }, segc, s.u.seg, z);
+            if (threadIdx.x == 0) s.carry[f] = cr.a;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        const int f = sl + i * GE::SUB;
+        Seg2 run = segCombine(Seg2{s.carry[f], 0.0, 0, 0}, s.scan[sg * BP + f]);
+        for (int64_t p = p0; p < p1; ++p) {
+            run = segCombine(run, Seg2{P.denom[p * BP + f], 0.0, (p > 0 && P.pseg[p] != P.pseg[p - 1]) ? 1 : 0, 0});
+            P.Dacc[p * BP + f] = run.a;
+        }
+    }
+    __syncthreads();
+    // ---- backward: R1, R2 = reverse segmented sums of nE/D, nE/D^2
+    auto bval = [&](int64_t p, int f) {
+        const double nw = P.nW[p * BP + f];
+        const double d = P.Dacc[p * BP + f];
+        const double r = nw == 0.0 ? 0.0 : nw / d;
+        return Seg2{r, nw == 0.0 ? 0.0 : r / d, (p == N - 1 || P.pseg[p + 1] != P.pseg[p]) ? 1 : 0, 0};
+    };
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        const int f = sl + i * GE::SUB;
+        Seg2 agg = z;
+        for (int64_t p = p1 - 1; p >= p0; --p) agg = segCombine(agg, bval(p, f));
+        s.scan[sg * BP + f] = agg;
+    }
+    __syncthreads();
+    slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
+    for (int f = threadIdx.x; f < BP; f += BLOCK) {
+        Seg2 run = z;
+        for (int q = GE::NSG - 1; q >= 0; --q) {
+            const Seg2 v = s.scan[q * BP + f];
+            s.scan[q * BP + f] = run;
+            run = segCombine(run, v);
+        }
+        double* o = slot + ((size_t)w * BP + f) * SLOTW;
+        o[0] = run.a; o[1] = run.b; o[2] = run.f;
+    }
+    publish(P.epoch, ++seq);
+    if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+    {
+        const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
+        const int nAfter = P.G - 1 - w;
+        for (int f = 0; f < BP; ++f) {
+            const Seg2 cr = treeReduce<Seg2>(nAfter, [&](int q) {
+                const double* o = sl2 + ((size_t)(P.G - 1 - q) * BP + f) * SLOTW;
+                return Seg2{o[0], o[1], (int)o[2], 0};
+            }, segc, s.u.seg, z);
+            if (threadIdx.x == 0) { s.carry[f] = cr.a; s.carry2[f] = cr.b; }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        const int f = sl + i * GE::SUB;
+        Seg2 run = segCombine(Seg2{s.carry[f], s.carry2[f], 0, 0}, s.scan[sg * BP + f]);
+        for (int64_t p = p1 - 1; p >= p0; --p) {
+            run = segCombine(run, bval(p, f));
+            P.R1[p * BP + f] = run.a;
+            P.R2[p * BP + f] = run.b;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+}  // namespace res
+}  // namespace ccd
+
+namespace ccd {
+namespace res {
+
+enum ObjOp : int { OBJ_LOGLIK = 0, OBJ_GRADIENT = 1, OBJ_LANGE = 2 };
+
+// This workgroup's share of the objective of every fit, into s.mon[f].g.
+template <typename Real, int MODEL, int BP>
+__device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, Shared<BP>& s) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    const int w = blockIdx.x;
+    double acc[GE::FPL];
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
+    {   // rows
+        const int64_t a = P.rowStart[w], b = P.rowStart[w + 1];
+        const int64_t c = (b - a + GE::NSG - 1) / GE::NSG;
+        int64_t n0 = a + sg * c, n1 = n0 + c;
+        if (n0 > b) n0 = b;
+        if (n1 > b) n1 = b;
+        for (int64_t k = n0; k < n1; ++k) {
+            const Real yk = P.y[k];
+            const int yi = (int)yk;
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) {
+                const int f = sl + i * GE::SUB;
+                if (f >= P.B) continue;
+                const double x = (double)P.xb[k * BP + f];
+                const double wt = (double)weightOf(P, k, f);
+                if (op == OBJ_GRADIENT) {
+                    acc[i] += x * (double)yk * wt;
+                    continue;
+                }
+                double cterm = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
+                if constexpr (MODEL == M_LOGISTIC) cterm -= log(1.0 + (double)dexp((Real)x));
+                if constexpr (MODEL == M_POISSON) cterm -= (double)dexp((Real)x);
+                acc[i] += cterm * wt;
+            }
+        }
+    }
+    if constexpr (GROUPED) {
+        if (op != OBJ_GRADIENT) {
+            const int64_t a = P.pidStart[w], b = P.pidStart[w + 1];
+            const int64_t c = (b - a + GE::NSG - 1) / GE::NSG;
+            int64_t n0 = a + sg * c, n1 = n0 + c;
+            if (n0 > b) n0 = b;
+            if (n1 > b) n1 = b;
+            const double* den = (MODEL == M_COX) ? P.Dacc : P.denom;
+            for (int64_t p = n0; p < n1; ++p) {
+#pragma unroll
+                for (int i = 0; i < GE::FPL; ++i) {
+                    const int f = sl + i * GE::SUB;
+                    if (f >= P.B) continue;
+                    const double nw = P.nW[p * BP + f];
+                    if (nw != 0.0) acc[i] -= nw * log(den[p * BP + f]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        const int f = sl + i * GE::SUB;
+        s.mon[sg * BP + f] = Mon{acc[i], 0, 0, 0, 0, 0};
+    }
+    __syncthreads();
+    combineSubgroups<BP>(s.mon);
+}
+
+// Per-pid initial state of this workgroup: nE, denominators, CLR risk ratios.
+template <typename Real, int MODEL, int BP>
+__device__ __forceinline__ void initPids(const Params<Real>& P) {
+    using GE = Geo<BP>;
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    const int w = blockIdx.x;
+    const int64_t a = P.pidStart[w], b = P.pidStart[w + 1];
+    const int64_t c = (b - a + GE::NSG - 1) / GE::NSG;
+    int64_t n0 = a + sg * c, n1 = n0 + c;
+    if (n0 > b) n0 = b;
+    if (n1 > b) n1 = b;
+    for (int64_t p = n0; p < n1; ++p) {
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            const int f = sl + i * GE::SUB;
+            double dn = 0.0, nw = 0.0;
+            if (f < P.B) {
+                for (int k = P.pstart[p]; k < P.pstart[p + 1]; ++k) {
+                    const Real wt = weightOf(P, (int64_t)k, f);
+                    Real e = dexp(P.xb[(int64_t)k * BP + f]);
+                    if constexpr (MODEL == M_COX) e *= wt;
+                    dn += (double)e;
+                    nw += (double)P.y[k] * (double)wt;
+                }
+            }
+            const size_t q = (size_t)p * BP + f;
+            P.denom[q] = dn;
+            P.nW[q] = nw;
+            if constexpr (MODEL == M_CLR) {
+                const double r = nw == 0.0 ? 0.0 : nw / dn;
+                P.R1[q] = r;
+                P.R2[q] = nw == 0.0 ? 0.0 : r / dn;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The persistent findMode kernel (CyclicCoordinateDescent.cpp:909-1216 for every fit).
+template <typename Real, int MODEL, int BP>
+__global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    __shared__ Shared<BP> s;
+    const int w = blockIdx.x;
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    const unsigned long long deadline = P.deadlineTicks;   // per-wait limit
+    unsigned seq = 0;
+    const bool own = P.allRead != 0;   // control state in LDS (all-read) or global (owner mode)
+    for (int f = threadIdx.x; f < BP; f += BLOCK) {
+        s.iter[f] = 0;
+        s.done[f] = f >= P.B ? 1 : 0;
+        s.flag[f] = 0;
+        s.last[f] = 0.0;
+    }
+    __syncthreads();
+    // per-fit control pointers (LDS copy in all-read mode; owner's global copy otherwise)
+    int* cIter = own ? s.iter : P.iters;
+    int* cFlag = own ? s.flag : P.flag;
+    double* cLast = own ? s.last : P.ll;
+    int* cDone = own ? s.done : P.done;
+
+    if constexpr (GROUPED) {
+        initPids<Real, MODEL, BP>(P);
+        if constexpr (MODEL == M_COX) {
+            // the pass above wrote only this workgroup's pids; the scan reads them all
+            if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+        }
+    }
+    const int objOp = P.convType == CCD_CONV_GRADIENT ? OBJ_GRADIENT
+                    : (P.convType == CCD_CONV_LANGE ? OBJ_LANGE : OBJ_LOGLIK);
+    const bool addFixed = (MODEL == M_POISSON) && objOp != OBJ_GRADIENT;
+    // log prior of fit f (JointPrior.h:123-131); read after the epoch that follows the
+    // last beta write of the cycle, so every workgroup sees the same beta
+    auto logPrior = [&](int f) {
+        double lp = 0.0;
+        for (int jj = 0; jj < P.J; ++jj)
+            lp += logDensityDev(P.exclude[jj] ? 0 : P.priorType, P.var[f], P.beta[(size_t)f * P.J + jj]);
+        return lp;
+    };
+    // initial objective -> lastObj
+    objectivePartial<Real, MODEL, BP>(P, objOp, s);
+    writeSlots<Real, BP>(P, seq, s);
+    if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
+            const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
+            cLast[f] = obj;
+            cIter[f] = 0;
+            cFlag[f] = 0;
+            cDone[f] = 0;
+            o[0] = 0.0;
+        }))
+        return;
+
+    bool allDone = false;
+    while (!allDone) {
+        for (int j = 0; j < P.J; ++j) {
+            if (P.fixed[j]) continue;
+            const int fm = P.fmt[j];
+            const int64_t nnzj = (fm == F_INDICATOR || fm == F_SPARSE) ? P.colPtr[j + 1] - P.colPtr[j] : P.K;
+            const int ptype = P.exclude[j] ? 0 : P.priorType;
+            if (own)
+                for (int f = threadIdx.x; f < P.B; f += BLOCK) {
+                    s.beta[f] = P.beta[(size_t)f * P.J + j];
+                    s.hd[f] = P.hDelta[(size_t)f * P.J + j];
+                }
+            // step of fit f from its reduced column pass: ccdUpdateBeta (:1394-1413),
+            // prior (CovariatePrior.h), applyBounds (:1503-1520)
+            auto step = [&](int f, const Mon& m, double* o) {
+                double g = 0.0, h = 0.0;
+                if (nnzj != 0) {
+                    const double xjy = P.xjyW ? P.xjyW[j]
+                                     : P.xjyTot[j] - ((P.xjyFold && P.held[f] >= 0) ? P.xjyFold[(size_t)j * P.F + P.held[f]] : 0.0);
+                    g = m.g - xjy;
+                    h = GROUPED ? m.h - m.l : m.h;
+                }
+                if (cDone[f]) { o[0] = 0.0; return; }
+                const double b = own ? s.beta[f] : P.beta[(size_t)f * P.J + j];
+                double hd = own ? s.hd[f] : P.hDelta[(size_t)f * P.J + j];
+                if (h < 0.0) { g = 0.0; h = 0.0; }
+                double delta = priorDeltaDev(ptype, P.var[f], g, h, b);
+                if (delta < -hd) delta = -hd;
+                else if (delta > hd) delta = hd;
+                hd = fmax(fmax(fabs(delta) * 2.0, hd / 2.0), 1E-3);
+                if (!own || w == 0) {
+                    P.hDelta[(size_t)f * P.J + j] = hd;
+                    if (delta != 0.0) P.beta[(size_t)f * P.J + j] = b + delta;
+                }
+                o[0] = delta;
+            };
+            if (nnzj == 0) {
+                // g = h = 0 without a pass (ModelSpecifics.hpp:720-723); identical everywhere
+                if (threadIdx.x == 0)
+                    for (int f = 0; f < P.B; ++f) {
+                        if (own) step(f, monZero(), s.out + 4 * f);
+                        else s.out[4 * f] = 0.0;
+                    }
+                __syncthreads();
+                if (!own) {
+                    // owner mode keeps beta/hDelta with the owners; an empty column still steps them
+                    if (threadIdx.x == 0)
+                        for (int f = w; f < P.B; f += P.G) {
+                            double o[4];
+                            step(f, monZero(), o);
+                            s.out[4 * f] = o[0];
+                        }
+                    // deltas of other owners are not needed: an empty column changes no row
+                    __syncthreads();
+                }
+                continue;
+            }
+            int64_t a, b, c0;
+            colRange(P, j, w, a, b, c0);
+            Mon acc[GE::FPL];
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) acc[i] = monZero();
+            switch (fm) {
+            case F_INDICATOR: gradPass<Real, MODEL, BP, F_INDICATOR>(P, j, a, b, c0, acc); break;
+            case F_SPARSE: gradPass<Real, MODEL, BP, F_SPARSE>(P, j, a, b, c0, acc); break;
+            case F_DENSE: gradPass<Real, MODEL, BP, F_DENSE>(P, j, a, b, c0, acc); break;
+            default: gradPass<Real, MODEL, BP, F_INTERCEPT>(P, j, a, b, c0, acc); break;
+            }
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + sl + i * GE::SUB] = acc[i];
+            __syncthreads();
+            combineSubgroups<BP>(s.mon);
+            writeSlots<Real, BP>(P, seq, s);
+            if (!reduceFits<Real, BP>(P, seq, deadline, s, step)) return;
+            __shared__ double sDelta[BP];
+            int any = 0;
+            for (int f = threadIdx.x; f < BP; f += BLOCK) {
+                const double d = f < P.B ? s.out[4 * f] : 0.0;
+                sDelta[f] = d;
+                any |= (d != 0.0);
+            }
+            if (__syncthreads_or(any)) {
+                switch (fm) {
+                case F_INDICATOR: updatePass<Real, MODEL, BP, F_INDICATOR>(P, j, a, b, sDelta); break;
+                case F_SPARSE: updatePass<Real, MODEL, BP, F_SPARSE>(P, j, a, b, sDelta); break;
+                case F_DENSE: updatePass<Real, MODEL, BP, F_DENSE>(P, j, a, b, sDelta); break;
+                default: updatePass<Real, MODEL, BP, F_INTERCEPT>(P, j, a, b, sDelta); break;
+                }
+                __syncthreads();
+                if constexpr (MODEL == M_COX) {
+                    if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+                }
+            }
+            if (w == 0 && threadIdx.x == 0) atomicAdd(P.steps, 1ull);
+        }
+        // convergence test of every fit (performCheckConvergence :839-906)
+        objectivePartial<Real, MODEL, BP>(P, objOp, s);
+        writeSlots<Real, BP>(P, seq, s);
+        if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
+                if (cDone[f]) { o[0] = 1.0; return; }
+                const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
+                cIter[f] += 1;
+                double conv;
+                bool ill = false;
+                if (obj != obj) { conv = 0.0; ill = true; }
+                else conv = fabs(obj - cLast[f]) / (fabs(obj) + 1.0);
+                cLast[f] = obj;
+                int done = 0;
+                if (P.tol > 0 && conv < P.tol) { cFlag[f] = ill ? CCD_ILLCONDITIONED : CCD_SUCCESS; done = 1; }
+                else if (cIter[f] == P.maxIt) { cFlag[f] = CCD_MAX_ITERATIONS; done = 1; }
+                cDone[f] = done;
+                o[0] = done;
+            }))
+            return;
+        int notDone = 0;
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) notDone |= (s.out[4 * f] == 0.0);
+        allDone = !__syncthreads_or(notDone);
+    }
+    // final log-likelihood of every fit (reported beside beta)
+    objectivePartial<Real, MODEL, BP>(P, OBJ_LOGLIK, s);
+    writeSlots<Real, BP>(P, seq, s);
+    if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
+            o[0] = m.g + ((MODEL == M_POISSON) ? P.fixedTermFit[f] : 0.0);
+            o[1] = cIter[f];
+            o[2] = cFlag[f];
+        }))
+        return;
+    if (w == 0)
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) {
+            P.ll[f] = s.out[4 * f];
+            P.iters[f] = (int)s.out[4 * f + 1];
+            P.flag[f] = (int)s.out[4 * f + 2];
+        }
+}
+
+}  // namespace res
+}  // namespace ccd

@@ -1,0 +1,144 @@
+"""GPU parity of the device-resident fit (one persistent launch per findMode), the
+batched lockstep fits and the cross-validation driver against the oracle."""
+import numpy as np
+import pytest
+
+import _cases
+import oracle
+from cyclops_amd import _abi
+from cyclops_amd.control import create_control, create_prior
+
+pytestmark = pytest.mark.gpu
+
+MODELS = ["lr", "pr", "clr", "cox"]
+
+
+@pytest.fixture(scope="module")
+def E():
+    from cyclops_amd import engine
+    assert engine.device_count() > 0, "no HIP device"
+    return engine
+
+
+def _prior(model):
+    return {"lr": create_prior("laplace", 0.5), "pr": create_prior("normal", 1.0),
+            "clr": create_prior("laplace", 1.0), "cox": create_prior("laplace", 0.5)}[model]
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-12)
+
+
+@pytest.mark.parametrize("conv", ["mittal", "gradient", "lange"])
+@pytest.mark.parametrize("model", MODELS)
+def test_resident_fp64_matches_oracle(E, model, conv):
+    d = _cases.ALL[model]()
+    print(f"[case] resident fp64 {model} {conv}", flush=True)
+    prior = _prior(model)
+    ctl = create_control(convergence_type=conv, tolerance=1e-8)
+    bo, ro = oracle.fit(d, prior, ctl)
+    bg, rg = E.fit_cyclops_model(d, prior, ctl, precision=64, mode="resident")
+    assert rg["return_flag"] == ro["return_flag"]
+    assert rg["iterations"] == ro["iterations"]
+    assert _rel(bg, bo) <= 1e-6, (bg, bo)
+    assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_resident_fp32_within_tolerance(E, model):
+    d = _cases.ALL[model]()
+    print(f"[case] resident fp32 {model}", flush=True)
+    prior = _prior(model)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    bo, ro = oracle.fit(d, prior, ctl)
+    bg, rg = E.fit_cyclops_model(d, prior, ctl, precision=32, mode="resident")
+    err = _rel(bg, bo)
+    print(f"resident fp32 {model}: coef rel err {err:.2e}, "
+          f"loglik rel err {abs(rg['log_likelihood'] - ro['log_likelihood']) / abs(ro['log_likelihood']):.2e}")
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_resident_weighted_fixed(E, model):
+    d = _cases.ALL[model]()
+    print(f"[case] resident weighted {model}", flush=True)
+    rng = np.random.default_rng(5)
+    if model == "clr":
+        keep = rng.random(d.pid.max() + 1) < 0.8
+        w = keep[d.pid].astype(float)
+    else:
+        w = (rng.random(d.K) < 0.8).astype(float)
+    fixed = np.zeros(d.J, dtype=np.uint8)
+    fixed[-1] = 1
+    prior = _prior(model)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    bo, ro = oracle.fit(d, prior, ctl, weights=w, fixed=fixed)
+    bg, rg = E.fit_cyclops_model(d, prior, ctl, weights=w, fixed=fixed, precision=64, mode="resident")
+    assert rg["iterations"] == ro["iterations"]
+    assert _rel(bg, bo) <= 1e-6
+    assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
+
+
+def _fold_setup(d, model, n_folds=3):
+    selector = _abi.SELECT_BY_PID if model == "clr" else _abi.SELECT_BY_ROW
+    pid = d.pid if model == "clr" else None
+    return oracle.assign_folds(d.K, pid, n_folds, selector, 7)
+
+
+@pytest.mark.parametrize("n_var", [1, 2, 7, 25, 90])
+@pytest.mark.parametrize("model", MODELS)
+def test_batched_fits_match_independent_oracle_fits(E, model, n_var):
+    """Fits of every (variance, held-out fold) pair in one launch equal the fits run alone."""
+    d = _cases.ALL[model]()
+    folds = 3
+    rf = _fold_setup(d, model, folds)
+    variances = np.exp(np.linspace(np.log(0.05), np.log(2.0), n_var))
+    var_list, held = [], []
+    for v in variances:
+        for f in range(folds):
+            var_list.append(v)
+            held.append(f)
+    if n_var == 1:   # a single unweighted fit through the batch entry point
+        var_list, held = [variances[0]], [-1]
+    n = len(var_list)
+    if n > 256:
+        var_list, held = var_list[:256], held[:256]
+        n = 256
+    print(f"[case] batched {model} n_fits={n}", flush=True)
+    ptype = "normal" if model == "pr" else "laplace"
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    eng = E.Engine(d, 64)
+    betas, res = eng.fit_batch(var_list, ptype, ctl, held_out_fold=held, row_fold=rf)
+    eng.close()
+    for i in range(0, n, max(1, n // 6)):
+        w = None if held[i] < 0 else (rf != held[i]).astype(float)
+        bo, ro = oracle.fit(d, create_prior(ptype, var_list[i]), ctl, weights=w)
+        assert res[i]["iterations"] == ro["iterations"], (i, res[i], ro)
+        assert _rel(betas[i], bo) <= 1e-6, (i, betas[i], bo)
+        assert abs(res[i]["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
+
+
+@pytest.mark.parametrize("model", ["lr", "clr"])
+def test_cross_validation_matches_oracle(E, model):
+    d = _cases.ALL[model]()
+    print(f"[case] cross-validation {model}", flush=True)
+    prior = create_prior("laplace", 1.0)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8, fold=4, grid_size=5,
+                         lower_limit=0.01, upper_limit=5.0, seed=123)
+    selector = _abi.SELECT_BY_PID if model == "clr" else _abi.SELECT_BY_ROW
+    ro = oracle.cross_validate(d, prior, ctl, selector)
+    eng = E.Engine(d, 64)
+    rg = eng.cross_validate(prior, ctl, selector)
+    eng.close()
+    assert np.allclose(rg["grid"], ro["grid"], rtol=1e-8), (rg["grid"], ro["grid"])
+    assert rg["best_variance"] == pytest.approx(ro["best_variance"], rel=1e-12)
+    assert _rel(rg["beta"], ro["beta"]) <= 1e-6
+    assert rg["n_fits"] == 20
+
+
+def test_fold_assignment_matches_oracle(E):
+    for sel, pid in [(_abi.SELECT_BY_ROW, None), (_abi.SELECT_BY_PID, np.repeat(np.arange(50), 3))]:
+        K = 150
+        a = E.assign_folds(K, pid, 10, sel, 42)
+        b = oracle.assign_folds(K, pid, 10, sel, 42)
+        assert np.array_equal(a, b)
