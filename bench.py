@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: fits/sec and fit latency of batched CCD fits on one MI355X, against the
+plain per-fit GPU path, on the Poisson workload (1,000,000 rows, 20,000 Zipf(1.1)
+indicator columns at 2% mean density, log time-at-risk offset, Normal prior).
+
+One step = one batched launch that fits B models of that data from a cold start:
+4 row folds (BY_ROW selection, each fit drops one fold) x B/4 prior variances on a
+log grid around 1.0, FP32 bulk arithmetic, convergence on relative log-likelihood
+change < 1e-6. The plain per-fit path (host CCD loop, one launch per engine call)
+and the single-fit resident kernel are timed on one member of the same mix.
+
+    python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--batch 64]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--plain-cycles", type=int, default=12,
+                    help="CCD cycles over which the plain per-fit path is timed (its per-cycle cost is constant)")
+    ap.add_argument("--folds", type=int, default=4)
+    ap.add_argument("--scale", type=float, default=1.0, help="workload scale (1.0 = full size)")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-cycles", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    os.environ.setdefault("CCD_PROGRESS", "1")   # one line per persistent launch
+    from cyclops_amd import _abi
+    from cyclops_amd import data as D
+    from cyclops_amd import engine as E
+    from cyclops_amd.control import create_control, create_prior
+
+    n_dev = E.device_count()
+    if n_dev < 1:
+        raise SystemExit("no HIP device")
+    t0 = time.perf_counter()
+    d, _ = D.make_config(1, scale=args.scale)
+    log(f"generated Poisson workload K={d.K} J={d.J} nnz={d.nnz} in {time.perf_counter() - t0:.1f}s")
+
+    folds = args.folds
+    n_var = max(1, args.batch // folds)
+    B = n_var * folds
+    grid = np.exp(np.linspace(math.log(0.1), math.log(10.0), n_var)) if n_var > 1 else np.array([1.0])
+    rf = E.assign_folds(d.K, None, folds, _abi.SELECT_BY_ROW, 1001)
+    var_list, held = [], []
+    for v in grid:
+        for f in range(folds):
+            var_list.append(float(v))
+            held.append(f)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-6)
+
+    t0 = time.perf_counter()
+    eng = E.Engine(d, _abi.FP32)
+    log(f"uploaded to HBM in {time.perf_counter() - t0:.1f}s")
+
+    # ---- batched steps -------------------------------------------------------
+    # a one-cycle call sizes the persistent workspace before timing
+    eng.fit_batch(var_list, "normal", create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=1),
+                  held_out_fold=held, row_fold=rf)
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        eng.fit_batch(var_list, "normal", ctl, held_out_fold=held, row_fold=rf)
+        log(f"warmup step {i}: {time.perf_counter() - t0:.2f}s")
+    eng.reset_kernel_stats()
+    step_times = []
+    betas = res = None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        betas, res = eng.fit_batch(var_list, "normal", ctl, held_out_fold=held, row_fold=rf)
+        step_times.append(time.perf_counter() - t0)
+        log(f"step {i}: {step_times[-1]:.2f}s ({B} fits)")
+    ks = eng.kernel_stats()
+    total = sum(step_times)
+    fits_per_sec = B * args.steps / total
+    iters = [r["iterations"] for r in res]
+    flags = [r["return_flag"] for r in res]
+    launch_ms = ks["device_ms"] / max(ks["launches"], 1)
+    bytes_per_launch = ks["algorithmic_bytes"] / max(ks["launches"], 1)
+    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9 if launch_ms > 0 else 0.0
+
+    # one member of the mix: variance closest to 1.0, fold 0
+    ref_i = int(np.argmin([abs(math.log(v)) + 10 * h for v, h in zip(var_list, held)]))
+    w_ref = (rf != held[ref_i]).astype(float)
+    prior_ref = create_prior("normal", var_list[ref_i])
+
+    # ---- resident single fit (converged) ----------------------------------------
+    t0 = time.perf_counter()
+    b_res, r_res = eng.fit(prior_ref, ctl, weights=w_ref, mode="resident")
+    t_resident = time.perf_counter() - t0
+    log(f"resident single fit: {t_resident:.2f}s, {r_res['iterations']} iterations")
+    scale_ref = max(np.max(np.abs(b_res)), 1e-12)
+    batched_vs_resident = float(np.max(np.abs(betas[ref_i] - b_res)) / scale_ref)
+
+    # ---- plain per-fit path ------------------------------------------------------
+    # Every cycle of the plain path performs the same launches (one gradient and one
+    # update per coordinate), so it is timed over `plain_cycles` cycles and
+    # extrapolated to the cycle count of the converged fit; the resident kernel run
+    # for the same cycles must give identical coefficients.
+    plain = None
+    if not args.skip_plain:
+        cyc = args.plain_cycles
+        ctl_cap = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=cyc)
+        t0 = time.perf_counter()
+        b_plain, r_plain = eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="plain")
+        t_plain = time.perf_counter() - t0
+        b_cap, r_cap = eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="resident")
+        per_cycle = t_plain / r_plain["iterations"]
+        t_fit = per_cycle * r_res["iterations"]
+        plain = {
+            "seconds_per_fit": t_fit,
+            "fits_per_sec": 1.0 / t_fit,
+            "seconds_per_cycle": per_cycle,
+            "timed_cycles": r_plain["iterations"],
+            "extrapolated_to_cycles": r_res["iterations"],
+            "plain_vs_resident_same_cycles_max_coef_rel_diff":
+                float(np.max(np.abs(b_plain - b_cap)) / max(np.max(np.abs(b_cap)), 1e-12)),
+        }
+        log(f"plain per-fit path: {per_cycle:.3f}s per cycle over {r_plain['iterations']} cycles")
+    eng.close()
+
+    # ---- CPU baseline: the oracle (FP64, one core) on the same data --------------
+    cpu = None
+    if not args.skip_cpu:
+        import oracle
+        cyc = args.cpu_cycles
+        sec, _ = oracle.time_cycles(d, prior_ref, ctl, cyc)
+        per_cycle = sec / cyc
+        it = r_res["iterations"]
+        cpu = {"value": 1.0 / (per_cycle * it), "unit": "fits/s", "cores": 1, "kind": "port",
+               "sample": f"{cyc} CCD cycle(s) of one fit of this workload in FP64 on one host core "
+                         f"({per_cycle:.2f}s per cycle), extrapolated to the {it} cycles the fit needs"}
+        log(f"cpu baseline: {per_cycle:.2f}s per cycle")
+
+    lat = np.array(step_times) * 1e3
+    out = {
+        "metric": "fits/sec + p50/p99 fit latency vs plain per-fit GPU path at matched beta tol",
+        "value": fits_per_sec,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk (f64 prior step, reductions and convergence test)",
+        "data": "synthetic (seeded generator, no dataset files)",
+        "config": {"workload": "poisson_offset_K1e6_P2e4_zipf1.1_density0.02_normal_prior", "K": d.K, "J": d.J,
+                   "nnz": d.nnz, "batch_fits": B, "folds": folds, "variances": n_var,
+                   "convergence": "mittal 1e-6", "iterations_max": max(iters), "iterations_min": min(iters),
+                   "return_flags": sorted(set(flags))},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "note": "batched fits complete together; per-fit latency = step time"},
+        "resident_single_fit": {"seconds": t_resident, "fits_per_sec": 1.0 / t_resident,
+                                "iterations": r_res["iterations"],
+                                "batched_vs_resident_max_coef_rel_diff": batched_vs_resident},
+        "plain_per_fit": plain,
+        "speedup_vs_plain": (fits_per_sec / plain["fits_per_sec"]) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ccd::res::kFindMode<float, POISSON, BP>", "launch_ms": launch_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

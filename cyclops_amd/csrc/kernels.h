@@ -124,18 +124,33 @@ __device__ __forceinline__ void blockReduce(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Publish-then-count: true in the workgroup that arrives last. The device-scope
-// fences order the partial stores before the count and the partial loads after it.
+// Partials that another workgroup reads are stored and loaded with relaxed
+// agent-scope atomics (coherent per word across the XCDs, no whole-cache flush);
+// workgroup-scope fences order them around the arrival count.
+__device__ __forceinline__ void pst(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double pld(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void psti(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int pldi(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish-then-count: true in the workgroup that arrives last.
 __device__ __forceinline__ bool lastBlockArrive(unsigned* counter) {
     __shared__ int amLast;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(counter, 1u);
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         amLast = (prev == gridDim.x - 1);
     }
     __syncthreads();
-    if (amLast) __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return amLast != 0;
 }
 
@@ -180,14 +195,14 @@ __global__ __launch_bounds__(BLOCK) void kGradIndep(Col<Real> c, const Real* __r
     double v[2] = {g, h};
     blockReduce<2>(v);
     if (threadIdx.x == 0) {
-        parts[blockIdx.x].g = v[0];
-        parts[blockIdx.x].h = v[1];
+        pst(&parts[blockIdx.x].g, v[0]);
+        pst(&parts[blockIdx.x].h, v[1]);
     }
     if (lastBlockArrive(counter)) {
         double s[2] = {0.0, 0.0};
         for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) {
-            s[0] += parts[b].g;
-            s[1] += parts[b].h;
+            s[0] += pld(&parts[b].g);
+            s[1] += pld(&parts[b].h);
         }
         blockReduce<2>(s);
         if (threadIdx.x == 0) {
@@ -307,21 +322,21 @@ __global__ __launch_bounds__(BLOCK) void kGradGrouped(Col<Real> c, const Real* _
     blockReduce<4>(v);
     if (threadIdx.x == 0) {
         Partial& P = parts[blockIdx.x];
-        P.g = v[0];
-        P.h = v[1];
-        P.q = v[2];
-        P.m = v[3];
-        P.t = carry;
-        P.reset = seen;
+        pst(&P.g, v[0]);
+        pst(&P.h, v[1]);
+        pst(&P.q, v[2]);
+        pst(&P.m, v[3]);
+        pst(&P.t, carry);
+        psti(&P.reset, seen);
     }
     if (lastBlockArrive(counter)) {
         const int b = threadIdx.x;
         const bool act = b < (int)gridDim.x;
-        Seg2 a{act ? parts[b].t : 0.0, 0.0, act ? parts[b].reset : 0, 0};
+        Seg2 a{act ? pld(&parts[b].t) : 0.0, 0.0, act ? pldi(&parts[b].reset) : 0, 0};
         Seg2 tot;
         const Seg2 off = blockSegScanExcl(a, tot);
-        double s2[3] = {act ? parts[b].g : 0.0, act ? parts[b].h : 0.0,
-                        act ? parts[b].q + 2.0 * off.a * parts[b].m : 0.0};
+        double s2[3] = {act ? pld(&parts[b].g) : 0.0, act ? pld(&parts[b].h) : 0.0,
+                        act ? pld(&parts[b].q) + 2.0 * off.a * pld(&parts[b].m) : 0.0};
         blockReduce<3>(s2);
         if (threadIdx.x == 0) {
             out[0] = s2[0] - xjy;
@@ -546,10 +561,10 @@ __global__ __launch_bounds__(BLOCK) void kRowReduce(int64_t K, int64_t N, const 
     }
     double v[1] = {acc};
     blockReduce<1>(v);
-    if (threadIdx.x == 0) parts[blockIdx.x].g = v[0];
+    if (threadIdx.x == 0) pst(&parts[blockIdx.x].g, v[0]);
     if (lastBlockArrive(counter)) {
         double s[1] = {0.0};
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) s[0] += parts[b].g;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK) s[0] += pld(&parts[b].g);
         blockReduce<1>(s);
         if (threadIdx.x == 0) {
             out[0] = s[0] + add;

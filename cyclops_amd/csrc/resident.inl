@@ -7,6 +7,7 @@
 #include <chrono>
 #include <numeric>
 #include <cstdlib>
+#include <cstdio>
 
 namespace ccd {
 
@@ -118,11 +119,13 @@ struct ResWork {
     unsigned char *exclude = nullptr, *fixed = nullptr;
     double *xjyTot = nullptr, *xjyFold = nullptr, *xjyW = nullptr, *fixedTerm = nullptr;
     int *held = nullptr, *rowFold = nullptr;
-    int *flag = nullptr, *iters = nullptr, *done = nullptr;
+    int *flag = nullptr, *iters = nullptr, *done = nullptr, *ctlIter = nullptr, *ctlFlag = nullptr;
+    double* ctlLast = nullptr;
     double* ll = nullptr;
     unsigned* epoch = nullptr;
     double *slots = nullptr, *bcast = nullptr;
     int* err = nullptr;
+    int* allDone = nullptr;
     unsigned long long* steps = nullptr;
     double* stats = nullptr;
     Real* kw = nullptr;
@@ -131,8 +134,8 @@ struct ResWork {
         dfree(xb); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold);
-        dfree(flag); dfree(iters); dfree(done); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
-        dfree(err); dfree(steps); dfree(stats); dfree(kw);
+        dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
+        dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw);
         B = BP = 0;
     }
     void freeAll() {
@@ -154,14 +157,18 @@ void EngineT<Real>::releaseResident() {
 
 // Workgroup count of the persistent launch: enough rows per workgroup to amortise
 // the per-coordinate grid exchange, never more workgroups than compute units.
-static int chooseWorkgroups(int64_t K, int device) {
+// One or two fits are bound by the latency of the per-coordinate exchange, which
+// grows with the number of workgroups; batches are bound by bandwidth and use the
+// whole chip.
+static int chooseWorkgroups(int64_t K, int B, int device) {
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* e = std::getenv("CCD_RESIDENT_WG")) {
         const int v = std::atoi(e);
         if (v > 0) return std::min(v, std::min(cus, res::MAXG));
     }
-    int64_t g = (K + 4095) / 4096;
+    const int64_t rowsPerWG = B <= 2 ? 16384 : 4096;
+    int64_t g = (K + rowsPerWG - 1) / rowsPerWG;
     if (g < 1) g = 1;
     if (g > cus) g = cus;
     if (g > res::MAXG) g = res::MAXG;
@@ -190,7 +197,8 @@ static void launchByBP(int BP, const res::Params<Real>& P, hipStream_t s) {
 template <typename Real>
 static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
                         const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
-                        int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps) {
+                        int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps,
+                        int& nLaunches) {
     auto& D = *E.D;
     hipStream_t s = E.s;
     const int64_t K = D.K;
@@ -199,7 +207,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     const bool grouped = (m == M_CLR || m == M_COX);
     if (!E.res) E.res = new ResWork<Real>();
     auto& W = *static_cast<ResWork<Real>*>(E.res);
-    const int G = chooseWorkgroups(K, D.device);
+    const int G = chooseWorkgroups(K, B, D.device);
     const int BP = padFits(B);
     if (BP > 256) throw std::invalid_argument("at most 256 fits per batched launch");
     // ---- structure for this G -------------------------------------------------
@@ -300,13 +308,17 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         W.flag = dalloc<int>(B, Bb);
         W.iters = dalloc<int>(B, Bb);
         W.done = dalloc<int>(B, Bb);
+        W.ctlIter = dalloc<int>(B, Bb);
+        W.ctlFlag = dalloc<int>(B, Bb);
+        W.ctlLast = dalloc<double>(B, Bb);
         W.ll = dalloc<double>(B, Bb);
         W.epoch = dalloc<unsigned>(G, Bb);
         W.slots = dalloc<double>((size_t)2 * G * BP * res::SLOTW, Bb);
         W.bcast = dalloc<double>((size_t)2 * BP * 4, Bb);
         W.err = dalloc<int>(1, Bb);
+        W.allDone = dalloc<int>(1, Bb);
         W.steps = dalloc<unsigned long long>(1, Bb);
-        W.stats = dalloc<double>(4, Bb);
+        W.stats = dalloc<double>(8, Bb);
         W.kw = dalloc<Real>(K, Bb);
     }
     // ---- inputs ---------------------------------------------------------------
@@ -383,6 +395,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     CCD_HIP(hipMemsetAsync(W.epoch, 0, G * sizeof(unsigned), s));
     CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
     CCD_HIP(hipMemsetAsync(W.steps, 0, sizeof(unsigned long long), s));
+    CCD_HIP(hipMemsetAsync(W.stats, 0, 8 * sizeof(double), s));
     CCD_HIP(hipMemsetAsync(W.done, 0, B * sizeof(int), s));
     CCD_HIP(hipMemsetAsync(W.flag, 0, B * sizeof(int), s));
     CCD_HIP(hipMemsetAsync(W.iters, 0, B * sizeof(int), s));
@@ -405,6 +418,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     P.fixedTermFit = W.fixedTerm;
     P.priorType = a.prior_type; P.convType = a.convergence_type; P.tol = a.tolerance; P.maxIt = a.max_iterations;
     P.flag = W.flag; P.done = W.done; P.iters = W.iters; P.ll = W.ll;
+    P.ctlIter = W.ctlIter; P.ctlFlag = W.ctlFlag; P.ctlLast = W.ctlLast;
     P.epoch = W.epoch; P.slots = W.slots; P.bcast = W.bcast; P.err = W.err;
     int rateKHz = 0;
     CCD_HIP(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, D.device));
@@ -412,46 +426,61 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     P.deadlineTicks = (unsigned long long)rateKHz * 1000ull * 20ull;   // 20 s for one grid exchange
     P.allRead = (B <= 2) ? 1 : 0;
     P.steps = W.steps;
-    // ---- launch -----------------------------------------------------------------
+    P.stats = W.stats;
+    P.allDoneOut = W.allDone;
+    // ---- launches: each runs up to `budget` cycles, state stays in HBM -----------
+    int budget = 32;
+    if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) budget = std::max(1, std::atoi(e));
+    const bool progress = std::getenv("CCD_PROGRESS") != nullptr;
+    const auto tWall = std::chrono::steady_clock::now();
     CCD_HIP(hipEventRecord(E.ev0, s));
-    switch (m) {
-    case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, P, s); break;
-    case M_POISSON: launchByBP<Real, M_POISSON>(BP, P, s); break;
-    case M_CLR: launchByBP<Real, M_CLR>(BP, P, s); break;
-    default: launchByBP<Real, M_COX>(BP, P, s); break;
+    int done = 0, launches = 0;
+    while (!done) {
+        CCD_HIP(hipMemsetAsync(W.epoch, 0, G * sizeof(unsigned), s));
+        CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
+        P.resume = launches > 0 ? 1 : 0;
+        P.cycleBudget = budget;
+        switch (m) {
+        case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, P, s); break;
+        case M_POISSON: launchByBP<Real, M_POISSON>(BP, P, s); break;
+        case M_CLR: launchByBP<Real, M_CLR>(BP, P, s); break;
+        default: launchByBP<Real, M_COX>(BP, P, s); break;
+        }
+        CCD_HIP(hipGetLastError());
+        int hs[2] = {0, 0};
+        CCD_HIP(hipMemcpyAsync(&hs[0], W.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        CCD_HIP(hipMemcpyAsync(&hs[1], W.allDone, sizeof(int), hipMemcpyDeviceToHost, s));
+        CCD_HIP(hipStreamSynchronize(s));
+        if (hs[0]) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
+        done = hs[1];
+        ++launches;
+        if (progress) {
+            std::vector<int> it(B);
+            CCD_HIP(hipMemcpy(it.data(), W.ctlIter, B * sizeof(int), hipMemcpyDeviceToHost));
+            std::fprintf(stderr, "[ccd] launch %d: %d cycles done, %s (%.1fs)\n", launches, launches * budget,
+                         done ? "all fits finished" : "fits still running",
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - tWall).count());
+        }
     }
-    CCD_HIP(hipGetLastError());
     CCD_HIP(hipEventRecord(E.ev1, s));
     CCD_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     CCD_HIP(hipEventElapsedTime(&ms, E.ev0, E.ev1));
     deviceMs = ms;
-    int errFlag = 0;
+    nLaunches = launches;
     unsigned long long st = 0;
-    CCD_HIP(hipMemcpy(&errFlag, W.err, sizeof(int), hipMemcpyDeviceToHost));
     CCD_HIP(hipMemcpy(&st, W.steps, sizeof(st), hipMemcpyDeviceToHost));
-    if (errFlag) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
     steps = (double)st;
     CCD_HIP(hipMemcpy(betas, W.beta, (size_t)B * J * sizeof(double), hipMemcpyDeviceToHost));
     CCD_HIP(hipMemcpy(flags, W.flag, B * sizeof(int), hipMemcpyDeviceToHost));
     CCD_HIP(hipMemcpy(iters, W.iters, B * sizeof(int), hipMemcpyDeviceToHost));
     CCD_HIP(hipMemcpy(lls, W.ll, B * sizeof(double), hipMemcpyDeviceToHost));
-    // compulsory bytes of the column passes actually run (per coordinate step)
-    double colBytes = 0.0;
-    for (int j = 0; j < J; ++j) {
-        if (fix[j]) continue;
-        const int f = D.fmt[j];
-        const double n = (double)((f == F_INDICATOR || f == F_SPARSE) ? D.colPtr[j + 1] - D.colPtr[j] : K);
-        double b = n * B * sizeof(Real);
-        if (f == F_INDICATOR || f == F_SPARSE) b += n * 4.0;
-        if (f == F_SPARSE || f == F_DENSE) b += n * sizeof(Real);
-        if (rowFold || kwHost) b += n * 4.0;
-        if (grouped) b += n * (8.0 + 16.0 * B);
-        colBytes += b;
-    }
-    int maxIt = 0;
-    for (int f = 0; f < B; ++f) maxIt = std::max(maxIt, iters[f]);
-    algoBytes = colBytes * maxIt;
+    double st8[8];
+    CCD_HIP(hipMemcpy(st8, W.stats, sizeof(st8), hipMemcpyDeviceToHost));
+    algoBytes = st8[0];
+    if (std::getenv("CCD_PHASES"))
+        std::fprintf(stderr, "[ccd] G=%d B=%d BP=%d steps=%llu  ms: column pass %.1f  exchange %.1f  update %.1f  objective %.1f  (kernel %.1f)\n",
+                     G, B, BP, st, st8[1] / rateKHz, st8[2] / rateKHz, st8[3] / rateKHz, st8[4] / rateKHz, (double)ms);
 }
 
 template <typename Real>
@@ -467,12 +496,13 @@ void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_res
     double var = a.variance;
     for (;;) {
         double ms = 0, bytes = 0, steps = 0;
+        int nl = 0;
         runResident<Real>(*this, a, 1, &var, nullptr, nullptr, 0, a.weights, &initBound, b.data(), &flag, &it, &ll,
-                          ms, bytes, steps);
+                          ms, bytes, steps, nl);
         ++count;
         if (timing) {
             counters.deviceMs += ms;
-            counters.launches += 1;
+            counters.launches += nl;
             counters.bytes += bytes;
             counters.coordinates += steps;
         }
@@ -537,11 +567,12 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
                 std::copy(betas + (size_t)f * J, betas + (size_t)(f + 1) * J, bb.begin() + (size_t)i * J);
             }
             double ms = 0, bytes = 0, steps = 0;
+            int nl = 0;
             runResident<Real>(*this, a, B, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(), bb.data(),
-                              fl.data(), it.data(), l.data(), ms, bytes, steps);
+                              fl.data(), it.data(), l.data(), ms, bytes, steps, nl);
             if (timing) {
                 counters.deviceMs += ms;
-                counters.launches += 1;
+                counters.launches += nl;
                 counters.bytes += bytes;
                 counters.coordinates += steps;
             }

@@ -87,6 +87,12 @@ struct Params {
     // results
     int* flag;            // [B]
     int* done;            // [B] owner-mode convergence state
+    int* ctlIter;         // [B]
+    int* ctlFlag;         // [B]
+    double* ctlLast;      // [B]
+    int resume;           // continue a run started by an earlier launch
+    int cycleBudget;      // cycles this launch may run
+    int* allDoneOut;      // set when every fit has finished
     int* iters;           // [B]
     double* ll;           // [B]
     // sync / scratch
@@ -98,17 +104,29 @@ struct Params {
     int allRead;
     // counters
     unsigned long long* steps;
+    double* stats;        // [0] compulsory bytes moved; [1..5] phase ticks of workgroup 0
 };
 
 constexpr int SLOTW = 6;
 
 __device__ __forceinline__ unsigned long long now() { return wall_clock64(); }
 
-// Store-release of this workgroup's epoch after every thread's writes are fenced.
+// Cross-workgroup words (epochs, slots, broadcasts, beta and bound state) are read
+// and written only with relaxed agent-scope atomics, which are coherent across the
+// XCDs per word without flushing or invalidating whole L2 caches; workgroup-scope
+// fences order a workgroup's own accesses around the epoch store and load.
+// Per-row and per-pid state is touched only by the workgroup that owns it.
+__device__ __forceinline__ void ast(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ald(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void publish(unsigned* epoch, unsigned seq) {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&epoch[blockIdx.x], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_store(&epoch[blockIdx.x], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wait until every workgroup has published `seq`. Returns false on timeout or a
@@ -119,7 +137,7 @@ __device__ __forceinline__ bool waitAll(const unsigned* epoch, unsigned seq, int
     for (;;) {
         int ok = 1;
         for (int w = threadIdx.x; w < G; w += BLOCK)
-            ok &= (__hip_atomic_load(&epoch[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= seq) ? 1 : 0;
+            ok &= (__hip_atomic_load(&epoch[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seq) ? 1 : 0;
         if (__syncthreads_and(ok)) break;
         int bad = 0;
         if (threadIdx.x == 0) {
@@ -130,9 +148,9 @@ __device__ __forceinline__ bool waitAll(const unsigned* epoch, unsigned seq, int
             }
         }
         if (__syncthreads_or(bad)) return false;
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return true;
 }
 
@@ -202,10 +220,19 @@ __device__ __forceinline__ void subChunk(const Params<Real>& P, int64_t a, int64
     }
 }
 
+// Entries are fetched U at a time (row ids, values, weights, per-fit state and
+// per-pid ratios all in flight together) and then combined strictly in entry
+// order, so the arithmetic is the same as a one-entry-at-a-time walk.
+template <int BP>
+struct Unroll {
+    static constexpr int U = Geo<BP>::FPL >= 4 ? 4 : (BP >= 16 ? 16 : 8);
+};
+
 template <typename Real, int MODEL, int BP, int FMT>
 __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t a, int64_t b, int64_t c0,
-                                         Mon (&acc)[Geo<BP>::FPL]) {
+                                         Mon (&acc)[Geo<BP>::FPL], const int* active) {
     using GE = Geo<BP>;
+    constexpr int U = Unroll<BP>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
     int64_t n0, n1;
@@ -218,48 +245,80 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t a
             hasPrev = true;
         }
     }
-    for (int64_t e = n0; e < n1; ++e) {
-        const int64_t k = rowAt<Real, FMT>(P, e);
-        const Real x = xAt<Real, FMT>(P, j, e);
-        bool reset = false;
-        int p = 0;
-        if constexpr (GROUPED) {
-            const int sgm = P.seg[k];
-            reset = hasPrev && sgm != prevSeg;
-            prevSeg = sgm;
-            hasPrev = true;
-            p = P.pid[k];
+    bool on[GE::FPL];
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        const int f = sl + i * GE::SUB;
+        on[i] = f < P.B && active[f];
+    }
+    for (int64_t e = n0; e < n1; e += U) {
+        int64_t kk[U];
+        Real xx[U], ww[U][GE::FPL], xv[U][GE::FPL];
+        int sgm[U], pp[U];
+        double r1[U][GE::FPL], r2[U][GE::FPL];
+        bool val[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            val[u] = e + u < n1;
+            kk[u] = val[u] ? rowAt<Real, FMT>(P, e + u) : 0;
+            xx[u] = val[u] ? xAt<Real, FMT>(P, j, e + u) : Real(0);
+            if constexpr (GROUPED) {
+                sgm[u] = val[u] ? P.seg[kk[u]] : 0;
+                pp[u] = val[u] ? P.pid[kk[u]] : 0;
+            }
         }
 #pragma unroll
-        for (int i = 0; i < GE::FPL; ++i) {
-            const int f = sl + i * GE::SUB;
-            if (f >= P.B) continue;
-            const Real xv = P.xb[k * BP + f];
-            const Real ee = dexp(xv);
-            if constexpr (MODEL == M_LOGISTIC) {
-                const Real wt = weightOf(P, k, f);
-                const Real pr = ee / (Real(1) + ee);
-                acc[i].g += (double)(wt * pr * x);
-                acc[i].h += (double)(wt * x * x * pr * (Real(1) - pr));
-            } else if constexpr (MODEL == M_POISSON) {
-                const Real wt = weightOf(P, k, f);
-                acc[i].g += (double)(wt * ee * x);
-                acc[i].h += (double)(wt * ee * x * x);
-            } else {
-                Real uu = ee * x;
-                if constexpr (MODEL == M_COX) uu *= weightOf(P, k, f);
-                const double u = (double)uu;
-                const double r1 = P.R1[(size_t)p * BP + f], r2 = P.R2[(size_t)p * BP + f];
-                if (reset) {
-                    acc[i].t = 0.0;
-                    acc[i].r = 1.0;
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) {
+                const int f = sl + i * GE::SUB;
+                const bool go = val[u] && on[i];
+                xv[u][i] = go ? P.xb[kk[u] * BP + f] : Real(0);
+                if constexpr (MODEL != M_CLR) ww[u][i] = go ? weightOf(P, kk[u], f) : Real(0);
+                if constexpr (GROUPED) {
+                    const size_t q = (size_t)pp[u] * BP + f;
+                    r1[u][i] = go ? P.R1[q] : 0.0;
+                    r2[u][i] = go ? P.R2[q] : 0.0;
                 }
-                const double s = acc[i].t + u;
-                acc[i].l += r2 * u * (2.0 * s - u);
-                if (acc[i].r == 0.0) acc[i].m += r2 * u;
-                acc[i].t = s;
-                acc[i].g += u * r1;
-                acc[i].h += u * (double)x * r1;
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!val[u]) break;
+            const Real x = xx[u];
+            bool reset = false;
+            if constexpr (GROUPED) {
+                reset = hasPrev && sgm[u] != prevSeg;
+                prevSeg = sgm[u];
+                hasPrev = true;
+            }
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) {
+                if (!on[i]) continue;
+                const Real ee = dexp(xv[u][i]);
+                if constexpr (MODEL == M_LOGISTIC) {
+                    const Real wt = ww[u][i];
+                    const Real pr = ee / (Real(1) + ee);
+                    acc[i].g += (double)(wt * pr * x);
+                    acc[i].h += (double)(wt * x * x * pr * (Real(1) - pr));
+                } else if constexpr (MODEL == M_POISSON) {
+                    const Real wt = ww[u][i];
+                    acc[i].g += (double)(wt * ee * x);
+                    acc[i].h += (double)(wt * ee * x * x);
+                } else {
+                    Real uu = ee * x;
+                    if constexpr (MODEL == M_COX) uu *= ww[u][i];
+                    const double uv = (double)uu;
+                    if (reset) {
+                        acc[i].t = 0.0;
+                        acc[i].r = 1.0;
+                    }
+                    const double sv = acc[i].t + uv;
+                    acc[i].l += r2[u][i] * uv * (2.0 * sv - uv);
+                    if (acc[i].r == 0.0) acc[i].m += r2[u][i] * uv;
+                    acc[i].t = sv;
+                    acc[i].g += uv * r1[u][i];
+                    acc[i].h += uv * (double)x * r1[u][i];
+                }
             }
         }
     }
@@ -269,50 +328,77 @@ template <typename Real, int MODEL, int BP, int FMT>
 __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t a, int64_t b,
                                            const double* sDelta) {
     using GE = Geo<BP>;
+    constexpr int U = Unroll<BP>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
     int64_t n0, n1;
     subChunk<Real, FMT, GROUPED>(P, a, b, sg, GE::NSG, n0, n1);
     double acc[GE::FPL];
+    double dl[GE::FPL];
 #pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
-    for (int64_t e = n0; e < n1; ++e) {
-        const int64_t k = rowAt<Real, FMT>(P, e);
-        const Real x = xAt<Real, FMT>(P, j, e);
+    for (int i = 0; i < GE::FPL; ++i) {
+        acc[i] = 0.0;
+        const int f = sl + i * GE::SUB;
+        dl[i] = f < P.B ? sDelta[f] : 0.0;
+    }
+    for (int64_t e = n0; e < n1; e += U) {
+        int64_t kk[U];
+        Real xx[U], xv[U][GE::FPL], ww[U][GE::FPL];
+        int pp[U], pn[U];
+        bool val[U];
 #pragma unroll
-        for (int i = 0; i < GE::FPL; ++i) {
-            const int f = sl + i * GE::SUB;
-            if (f >= P.B) continue;
-            const double d = sDelta[f];
-            if (d == 0.0) continue;
-            Real* px = P.xb + k * BP + f;
-            const Real old = *px;
-            const Real nv = old + (Real)d * x;
-            *px = nv;
+        for (int u = 0; u < U; ++u) {
+            val[u] = e + u < n1;
+            kk[u] = val[u] ? rowAt<Real, FMT>(P, e + u) : 0;
+            xx[u] = val[u] ? xAt<Real, FMT>(P, j, e + u) : Real(0);
             if constexpr (GROUPED) {
-                Real diff = dexp(nv) - dexp(old);
-                if constexpr (MODEL == M_COX) diff *= weightOf(P, k, f);
-                acc[i] += (double)diff;
+                pp[u] = val[u] ? P.pid[kk[u]] : -1;
+                pn[u] = (val[u] && e + u + 1 < n1) ? P.pid[rowAt<Real, FMT>(P, e + u + 1)] : -1;
             }
         }
-        if constexpr (GROUPED) {
-            const int p = P.pid[k];
-            const bool runEnd = (e + 1 >= n1) || P.pid[rowAt<Real, FMT>(P, e + 1)] != p;
-            if (runEnd) {
 #pragma unroll
-                for (int i = 0; i < GE::FPL; ++i) {
-                    const int f = sl + i * GE::SUB;
-                    if (f >= P.B || sDelta[f] == 0.0) continue;
-                    const size_t q = (size_t)p * BP + f;
-                    const double dn = P.denom[q] + acc[i];
-                    P.denom[q] = dn;
-                    if constexpr (MODEL == M_CLR) {
-                        const double nw = P.nW[q];
-                        const double r = nw == 0.0 ? 0.0 : nw / dn;
-                        P.R1[q] = r;
-                        P.R2[q] = nw == 0.0 ? 0.0 : r / dn;
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) {
+                const int f = sl + i * GE::SUB;
+                const bool go = val[u] && dl[i] != 0.0;
+                xv[u][i] = go ? P.xb[kk[u] * BP + f] : Real(0);
+                if constexpr (MODEL == M_COX) ww[u][i] = go ? weightOf(P, kk[u], f) : Real(0);
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!val[u]) break;
+#pragma unroll
+            for (int i = 0; i < GE::FPL; ++i) {
+                if (dl[i] == 0.0) continue;
+                const int f = sl + i * GE::SUB;
+                const Real old = xv[u][i];
+                const Real nv = old + (Real)dl[i] * xx[u];
+                P.xb[kk[u] * BP + f] = nv;
+                if constexpr (GROUPED) {
+                    Real diff = dexp(nv) - dexp(old);
+                    if constexpr (MODEL == M_COX) diff *= ww[u][i];
+                    acc[i] += (double)diff;
+                }
+            }
+            if constexpr (GROUPED) {
+                if (pn[u] != pp[u]) {   // last entry of a run of equal pid
+                    const int p = pp[u];
+#pragma unroll
+                    for (int i = 0; i < GE::FPL; ++i) {
+                        if (dl[i] == 0.0) continue;
+                        const int f = sl + i * GE::SUB;
+                        const size_t q = (size_t)p * BP + f;
+                        const double dn = P.denom[q] + acc[i];
+                        P.denom[q] = dn;
+                        if constexpr (MODEL == M_CLR) {
+                            const double nw = P.nW[q];
+                            const double r = nw == 0.0 ? 0.0 : nw / dn;
+                            P.R1[q] = r;
+                            P.R2[q] = nw == 0.0 ? 0.0 : r / dn;
+                        }
+                        acc[i] = 0.0;
                     }
-                    acc[i] = 0.0;
                 }
             }
         }
@@ -433,6 +519,7 @@ struct Shared {
     int iter[BP];
     int done[BP];
     int flag[BP];
+    int active[BP];   // fits still iterating (identical in every workgroup)
 };
 
 // Publish slots, wait, reduce per fit (all-read: every workgroup reduces every fit;
@@ -440,36 +527,48 @@ struct Shared {
 // s.out[4f..4f+3] holds fn's output for every fit in every workgroup.
 template <typename Real, int BP, class Fn>
 __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
-                                           Shared<BP>& s, Fn fn) {
+                                           Shared<BP>& s, Fn fn, bool ordered = true) {
     publish(P.epoch, ++seq);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     const double* slot = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
     auto comb = [](const Mon& a, const Mon& b) { return monCombine(a, b); };
+    // ordered combine of the G partials of fit f; plain sums (fixed shuffle tree)
+    // when the partials carry no segment state
+    auto reduceOne = [&](int f) -> Mon {
+        if (!ordered) {
+            double v[2] = {0.0, 0.0};
+            for (int w = threadIdx.x; w < P.G; w += BLOCK) {
+                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+                v[0] += ald(q);
+                v[1] += ald(q + 1);
+            }
+            blockReduce<2>(v);
+            return Mon{v[0], v[1], 0, 0, 0, 0};
+        }
+        return treeReduce<Mon>(P.G, [&](int w) {
+            const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+            return Mon{ald(q), ald(q + 1), ald(q + 2), ald(q + 3), ald(q + 4), ald(q + 5)};
+        }, comb, s.u.red, monZero());
+    };
     if (P.allRead) {
         for (int f = 0; f < P.B; ++f) {
-            const Mon m = treeReduce<Mon>(P.G, [&](int w) {
-                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
-                return Mon{q[0], q[1], q[2], q[3], q[4], q[5]};
-            }, comb, s.u.red, monZero());
+            const Mon m = reduceOne(f);
             if (threadIdx.x == 0) fn(f, m, s.out + 4 * f);
         }
         __syncthreads();
     } else {
         double* bc = P.bcast + (size_t)((seq + 1) & 1) * BP * 4;
         for (int f = blockIdx.x; f < P.B; f += P.G) {
-            const Mon m = treeReduce<Mon>(P.G, [&](int w) {
-                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
-                return Mon{q[0], q[1], q[2], q[3], q[4], q[5]};
-            }, comb, s.u.red, monZero());
+            const Mon m = reduceOne(f);
             if (threadIdx.x == 0) {
                 double o[4] = {0, 0, 0, 0};
                 fn(f, m, o);
-                for (int i = 0; i < 4; ++i) bc[f * 4 + i] = o[i];
+                for (int i = 0; i < 4; ++i) ast(&bc[f * 4 + i], o[i]);
             }
         }
         publish(P.epoch, ++seq);
         if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
-        for (int i = threadIdx.x; i < P.B * 4; i += BLOCK) s.out[i] = bc[i];
+        for (int i = threadIdx.x; i < P.B * 4; i += BLOCK) s.out[i] = ald(&bc[i]);
         __syncthreads();
     }
     return true;
@@ -477,12 +576,15 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
 
 // Write this workgroup's per-fit Mons (s.mon[f]) to the slots of the next epoch.
 template <typename Real, int BP>
-__device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, const Shared<BP>& s) {
+__device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, const Shared<BP>& s,
+                                           bool ordered = true) {
     double* slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
     for (int f = threadIdx.x; f < BP; f += BLOCK) {
         double* q = slot + ((size_t)blockIdx.x * BP + f) * SLOTW;
         const Mon& m = s.mon[f];
-        q[0] = m.g; q[1] = m.h; q[2] = m.l; q[3] = m.m; q[4] = m.t; q[5] = m.r;
+        ast(q, m.g);
+        ast(q + 1, m.h);
+        if (ordered) { ast(q + 2, m.l); ast(q + 3, m.m); ast(q + 4, m.t); ast(q + 5, m.r); }
     }
 }
 
@@ -520,7 +622,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
             run = segCombine(run, v);
         }
         double* o = slot + ((size_t)w * BP + f) * SLOTW;
-        o[0] = run.a; o[1] = run.b; o[2] = run.f;
+        ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
     }
     publish(P.epoch, ++seq);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
@@ -529,7 +631,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
         for (int f = 0; f < BP; ++f) {
             const Seg2 cr = treeReduce<Seg2>(w, [&](int q) {
                 const double* o = sl2 + ((size_t)q * BP + f) * SLOTW;
-                return Seg2{o[0], o[1], (int)o[2], 0};
+                return Seg2{ald(o), ald(o + 1), (int)ald(o + 2), 0};
             }, segc, s.u.seg, z);
             if (threadIdx.x == 0) s.carry[f] = cr.a;
         }
@@ -569,7 +671,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
             run = segCombine(run, v);
         }
         double* o = slot + ((size_t)w * BP + f) * SLOTW;
-        o[0] = run.a; o[1] = run.b; o[2] = run.f;
+        ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
     }
     publish(P.epoch, ++seq);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
@@ -579,7 +681,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
         for (int f = 0; f < BP; ++f) {
             const Seg2 cr = treeReduce<Seg2>(nAfter, [&](int q) {
                 const double* o = sl2 + ((size_t)(P.G - 1 - q) * BP + f) * SLOTW;
-                return Seg2{o[0], o[1], (int)o[2], 0};
+                return Seg2{ald(o), ald(o + 1), (int)ald(o + 2), 0};
             }, segc, s.u.seg, z);
             if (threadIdx.x == 0) { s.carry[f] = cr.a; s.carry2[f] = cr.b; }
         }
@@ -721,23 +823,27 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
     unsigned seq = 0;
     const bool own = P.allRead != 0;   // control state in LDS (all-read) or global (owner mode)
     for (int f = threadIdx.x; f < BP; f += BLOCK) {
-        s.iter[f] = 0;
-        s.done[f] = f >= P.B ? 1 : 0;
-        s.flag[f] = 0;
-        s.last[f] = 0.0;
+        const bool real = f < P.B;
+        s.iter[f] = (P.resume && real) ? P.ctlIter[f] : 0;
+        s.done[f] = !real ? 1 : (P.resume ? P.done[f] : 0);
+        s.flag[f] = (P.resume && real) ? P.ctlFlag[f] : 0;
+        s.last[f] = (P.resume && real) ? P.ctlLast[f] : 0.0;
+        s.active[f] = real && !s.done[f] ? 1 : 0;
     }
     __syncthreads();
     // per-fit control pointers (LDS copy in all-read mode; owner's global copy otherwise)
-    int* cIter = own ? s.iter : P.iters;
-    int* cFlag = own ? s.flag : P.flag;
-    double* cLast = own ? s.last : P.ll;
+    int* cIter = own ? s.iter : P.ctlIter;
+    int* cFlag = own ? s.flag : P.ctlFlag;
+    double* cLast = own ? s.last : P.ctlLast;
     int* cDone = own ? s.done : P.done;
 
     if constexpr (GROUPED) {
-        initPids<Real, MODEL, BP>(P);
-        if constexpr (MODEL == M_COX) {
-            // the pass above wrote only this workgroup's pids; the scan reads them all
-            if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+        if (!P.resume) {
+            initPids<Real, MODEL, BP>(P);
+            if constexpr (MODEL == M_COX) {
+                // the pass above wrote only this workgroup's pids; the scan reads them all
+                if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+            }
         }
     }
     const int objOp = P.convType == CCD_CONV_GRADIENT ? OBJ_GRADIENT
@@ -748,24 +854,31 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
     auto logPrior = [&](int f) {
         double lp = 0.0;
         for (int jj = 0; jj < P.J; ++jj)
-            lp += logDensityDev(P.exclude[jj] ? 0 : P.priorType, P.var[f], P.beta[(size_t)f * P.J + jj]);
+            lp += logDensityDev(P.exclude[jj] ? 0 : P.priorType, P.var[f], ald(&P.beta[(size_t)f * P.J + jj]));
         return lp;
     };
     // initial objective -> lastObj
-    objectivePartial<Real, MODEL, BP>(P, objOp, s);
-    writeSlots<Real, BP>(P, seq, s);
-    if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
-            const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
-            cLast[f] = obj;
-            cIter[f] = 0;
-            cFlag[f] = 0;
-            cDone[f] = 0;
-            o[0] = 0.0;
-        }))
-        return;
+    if (!P.resume) {
+        objectivePartial<Real, MODEL, BP>(P, objOp, s);
+        writeSlots<Real, BP>(P, seq, s, false);
+        if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
+                const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
+                cLast[f] = obj;
+                cIter[f] = 0;
+                cFlag[f] = 0;
+                cDone[f] = 0;
+                o[0] = 0.0;
+            }, false))
+            return;
+    }
 
     bool allDone = false;
-    while (!allDone) {
+    {
+        int act = 0;
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) act |= s.active[f];
+        allDone = !__syncthreads_or(act);
+    }
+    for (int cycle = 0; !allDone && cycle < P.cycleBudget; ++cycle) {
         for (int j = 0; j < P.J; ++j) {
             if (P.fixed[j]) continue;
             const int fm = P.fmt[j];
@@ -773,8 +886,8 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
             const int ptype = P.exclude[j] ? 0 : P.priorType;
             if (own)
                 for (int f = threadIdx.x; f < P.B; f += BLOCK) {
-                    s.beta[f] = P.beta[(size_t)f * P.J + j];
-                    s.hd[f] = P.hDelta[(size_t)f * P.J + j];
+                    s.beta[f] = ald(&P.beta[(size_t)f * P.J + j]);
+                    s.hd[f] = ald(&P.hDelta[(size_t)f * P.J + j]);
                 }
             // step of fit f from its reduced column pass: ccdUpdateBeta (:1394-1413),
             // prior (CovariatePrior.h), applyBounds (:1503-1520)
@@ -787,16 +900,16 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
                     h = GROUPED ? m.h - m.l : m.h;
                 }
                 if (cDone[f]) { o[0] = 0.0; return; }
-                const double b = own ? s.beta[f] : P.beta[(size_t)f * P.J + j];
-                double hd = own ? s.hd[f] : P.hDelta[(size_t)f * P.J + j];
+                const double b = own ? s.beta[f] : ald(&P.beta[(size_t)f * P.J + j]);
+                double hd = own ? s.hd[f] : ald(&P.hDelta[(size_t)f * P.J + j]);
                 if (h < 0.0) { g = 0.0; h = 0.0; }
                 double delta = priorDeltaDev(ptype, P.var[f], g, h, b);
                 if (delta < -hd) delta = -hd;
                 else if (delta > hd) delta = hd;
                 hd = fmax(fmax(fabs(delta) * 2.0, hd / 2.0), 1E-3);
                 if (!own || w == 0) {
-                    P.hDelta[(size_t)f * P.J + j] = hd;
-                    if (delta != 0.0) P.beta[(size_t)f * P.J + j] = b + delta;
+                    ast(&P.hDelta[(size_t)f * P.J + j], hd);
+                    if (delta != 0.0) ast(&P.beta[(size_t)f * P.J + j], b + delta);
                 }
                 o[0] = delta;
             };
@@ -821,23 +934,26 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
                 }
                 continue;
             }
+            const unsigned long long tA = now();
             int64_t a, b, c0;
             colRange(P, j, w, a, b, c0);
             Mon acc[GE::FPL];
 #pragma unroll
             for (int i = 0; i < GE::FPL; ++i) acc[i] = monZero();
             switch (fm) {
-            case F_INDICATOR: gradPass<Real, MODEL, BP, F_INDICATOR>(P, j, a, b, c0, acc); break;
-            case F_SPARSE: gradPass<Real, MODEL, BP, F_SPARSE>(P, j, a, b, c0, acc); break;
-            case F_DENSE: gradPass<Real, MODEL, BP, F_DENSE>(P, j, a, b, c0, acc); break;
-            default: gradPass<Real, MODEL, BP, F_INTERCEPT>(P, j, a, b, c0, acc); break;
+            case F_INDICATOR: gradPass<Real, MODEL, BP, F_INDICATOR>(P, j, a, b, c0, acc, s.active); break;
+            case F_SPARSE: gradPass<Real, MODEL, BP, F_SPARSE>(P, j, a, b, c0, acc, s.active); break;
+            case F_DENSE: gradPass<Real, MODEL, BP, F_DENSE>(P, j, a, b, c0, acc, s.active); break;
+            default: gradPass<Real, MODEL, BP, F_INTERCEPT>(P, j, a, b, c0, acc, s.active); break;
             }
 #pragma unroll
             for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + sl + i * GE::SUB] = acc[i];
             __syncthreads();
             combineSubgroups<BP>(s.mon);
-            writeSlots<Real, BP>(P, seq, s);
-            if (!reduceFits<Real, BP>(P, seq, deadline, s, step)) return;
+            const unsigned long long tB = now();
+            writeSlots<Real, BP>(P, seq, s, GROUPED);
+            if (!reduceFits<Real, BP>(P, seq, deadline, s, step, GROUPED)) return;
+            const unsigned long long tC = now();
             __shared__ double sDelta[BP];
             int any = 0;
             for (int f = threadIdx.x; f < BP; f += BLOCK) {
@@ -857,11 +973,36 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
                     if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
                 }
             }
-            if (w == 0 && threadIdx.x == 0) atomicAdd(P.steps, 1ull);
+            if (w == 0 && threadIdx.x == 0) {
+                const unsigned long long tD = now();
+                P.stats[1] += (double)(tB - tA);
+                P.stats[2] += (double)(tC - tB);
+                P.stats[3] += (double)(tD - tC);
+                atomicAdd(P.steps, 1ull);
+                // compulsory bytes of this coordinate (my algorithm): the column pass
+                // reads ids/values/weights once and xb of every active fit; the update
+                // reads and writes xb of every fit that moved
+                double nAct = 0, nUpd = 0;
+                for (int f = 0; f < P.B; ++f) {
+                    nAct += s.active[f] ? 1 : 0;
+                    nUpd += (sDelta[f] != 0.0) ? 1 : 0;
+                }
+                const double n = (double)nnzj;
+                const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
+                                         ((fm == F_SPARSE || fm == F_DENSE) ? sizeof(Real) : 0.0) +
+                                         ((P.rowFold || P.kw) ? 4.0 : 0.0));
+                double b = base + n * nAct * sizeof(Real) + (GROUPED ? n * (8.0 + 16.0 * nAct) : 0.0);
+                if (nUpd > 0) {
+                    b += base + n * nUpd * 2.0 * sizeof(Real) + (GROUPED ? n * (4.0 + 16.0 * nUpd) : 0.0);
+                    if (MODEL == M_COX) b += (double)P.N * P.B * 56.0;
+                }
+                P.stats[0] += b;
+            }
         }
         // convergence test of every fit (performCheckConvergence :839-906)
+        const unsigned long long tO = now();
         objectivePartial<Real, MODEL, BP>(P, objOp, s);
-        writeSlots<Real, BP>(P, seq, s);
+        writeSlots<Real, BP>(P, seq, s, false);
         if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
                 if (cDone[f]) { o[0] = 1.0; return; }
                 const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
@@ -876,20 +1017,41 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
                 else if (cIter[f] == P.maxIt) { cFlag[f] = CCD_MAX_ITERATIONS; done = 1; }
                 cDone[f] = done;
                 o[0] = done;
-            }))
+            }, false))
             return;
         int notDone = 0;
-        for (int f = threadIdx.x; f < P.B; f += BLOCK) notDone |= (s.out[4 * f] == 0.0);
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) {
+            const int act = (s.out[4 * f] == 0.0) ? 1 : 0;
+            s.active[f] = act;
+            notDone |= act;
+        }
         allDone = !__syncthreads_or(notDone);
+        if (w == 0 && threadIdx.x == 0) P.stats[4] += (double)(now() - tO);
+        if (w == 0 && threadIdx.x == 0) {   // objective pass: y, xb of every fit, per-pid terms
+            double nAct = 0;
+            for (int f = 0; f < P.B; ++f) nAct += 1;
+            P.stats[0] += (double)P.K * (sizeof(Real) * nAct + sizeof(Real) + (P.rowFold ? 4.0 : 0.0)) +
+                          (GROUPED ? (double)P.N * nAct * 16.0 : 0.0);
+        }
     }
+    if (own && w == 0) {   // persist the control state for a later launch
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) {
+            P.ctlIter[f] = s.iter[f];
+            P.ctlFlag[f] = s.flag[f];
+            P.ctlLast[f] = s.last[f];
+            P.done[f] = s.done[f];
+        }
+    }
+    if (w == 0 && threadIdx.x == 0) *P.allDoneOut = allDone ? 1 : 0;
+    if (!allDone) return;
     // final log-likelihood of every fit (reported beside beta)
     objectivePartial<Real, MODEL, BP>(P, OBJ_LOGLIK, s);
-    writeSlots<Real, BP>(P, seq, s);
+    writeSlots<Real, BP>(P, seq, s, false);
     if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
             o[0] = m.g + ((MODEL == M_POISSON) ? P.fixedTermFit[f] : 0.0);
             o[1] = cIter[f];
             o[2] = cFlag[f];
-        }))
+        }, false))
         return;
     if (w == 0)
         for (int f = threadIdx.x; f < P.B; f += BLOCK) {
