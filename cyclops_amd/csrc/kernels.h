@@ -140,9 +140,17 @@ __device__ __forceinline__ int pldi(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Wait until every global store this wave issued has been acknowledged. Agent-scope
+// stores are coherent once acknowledged, but stores to different addresses are
+// acknowledged in any order, and a workgroup-scope fence does not wait for them:
+// without this, a flag or counter written after a workgroup barrier can become
+// visible to another workgroup before the data stored ahead of the barrier.
+__device__ __forceinline__ void drainStores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Publish-then-count: true in the workgroup that arrives last.
 __device__ __forceinline__ bool lastBlockArrive(unsigned* counter) {
     __shared__ int amLast;
+    drainStores();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -482,7 +490,7 @@ __global__ __launch_bounds__(BLOCK) void kScanReduce(Op op, Seg2* parts) {
 }
 
 // Exclusive segmented scan of nb block aggregates, in place, one workgroup.
-__global__ __launch_bounds__(BLOCK) void kScanPartials(int64_t nb, Seg2* parts) {
+static __global__ __launch_bounds__(BLOCK) void kScanPartials(int64_t nb, Seg2* parts) {
     const int64_t per = (nb + BLOCK - 1) / BLOCK;
     const int64_t lo = (int64_t)threadIdx.x * per;
     Seg2 acc{0.0, 0.0, 0, 0};

@@ -80,16 +80,17 @@ __global__ __launch_bounds__(BLOCK) void kXjYFold(int J, int64_t K, int F, const
 
 // xb[k*BP + f] += beta_f * x_kj for every fit f (one column).
 template <typename Real, int FMT>
-__global__ __launch_bounds__(BLOCK) void kAxpyBatch(Col<Real> c, const double* __restrict__ beta, int B, int BP,
-                                                   Real* __restrict__ xb) {
+__global__ __launch_bounds__(BLOCK) void kAxpyBatch(Col<Real> c, const double* __restrict__ beta, int J, int j,
+                                                   int B, int Bt, int BP, int64_t K, Real* __restrict__ xb) {
+    // xb of fit f lives in team f / Bt, lane f % Bt: xb[(f/Bt)*K*BP + row*BP + f%Bt]
     const int64_t total = c.n * (int64_t)B;
     const int64_t stride = (int64_t)gridDim.x * BLOCK;
     for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < total; t += stride) {
         const int64_t e = t / B;
         const int f = (int)(t % B);
-        const double bf = beta[f];
+        const double bf = beta[(size_t)f * J + j];
         if (bf == 0.0) continue;
-        xb[rowOf<Real, FMT>(c, e) * BP + f] += (Real)bf * xOf<Real, FMT>(c, e);
+        xb[(size_t)(f / Bt) * K * BP + rowOf<Real, FMT>(c, e) * BP + f % Bt] += (Real)bf * xOf<Real, FMT>(c, e);
     }
 }
 
@@ -102,7 +103,7 @@ static int padFits(int B) {
 // Device workspace of the persistent kernel, kept across fits of one engine.
 template <typename Real>
 struct ResWork {
-    int G = 0, BP = 0, B = 0;
+    int G = 0, BP = 0, B = 0, nTeams = 0;
     int64_t N = 0;
     double bytes = 0.0;
     // structure (depends on G)
@@ -129,14 +130,15 @@ struct ResWork {
     unsigned long long* steps = nullptr;
     double* stats = nullptr;
     Real* kw = nullptr;
+    res::Params<Real>* teams = nullptr;
 
     void freeBatch() {
         dfree(xb); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
-        dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw);
-        B = BP = 0;
+        dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw); dfree(teams);
+        B = BP = nTeams = 0;
     }
     void freeAll() {
         freeBatch();
@@ -175,30 +177,59 @@ static int chooseWorkgroups(int64_t K, int B, int device) {
     return (int)g;
 }
 
-template <typename Real, int MODEL, int BP>
-static void launchFindMode(const res::Params<Real>& P, hipStream_t s) {
-    hipLaunchKernelGGL((res::kFindMode<Real, MODEL, BP>), dim3(P.G), dim3(BLOCK), 0, s, P);
-}
+// Layout of a persistent launch: nTeams teams of T workgroups; team t runs fits
+// [t*Bt, min((t+1)*Bt, B)) packed fit-minor (padded to BP lanes). Teams never wait
+// on each other; a team of T > 1 workgroups exchanges per coordinate, so all
+// nTeams*T workgroups must be co-resident when T > 1.
+struct Layout {
+    int nTeams = 1, T = 1, Bt = 1;
+};
 
-template <typename Real, int MODEL>
-static void launchByBP(int BP, const res::Params<Real>& P, hipStream_t s) {
-    switch (BP) {
-    case 1: launchFindMode<Real, MODEL, 1>(P, s); break;
-    case 4: launchFindMode<Real, MODEL, 4>(P, s); break;
-    case 16: launchFindMode<Real, MODEL, 16>(P, s); break;
-    case 64: launchFindMode<Real, MODEL, 64>(P, s); break;
-    case 256: launchFindMode<Real, MODEL, 256>(P, s); break;
-    default: throw std::invalid_argument("unsupported fit padding");
+// CCD_BATCH_LAYOUT="Bt,T" overrides the team shape of batches (B > 2).
+static Layout chooseLayout(int64_t K, int B, int device) {
+    Layout L;
+    int bt = 0, t = 0;
+    if (const char* e = std::getenv("CCD_BATCH_LAYOUT")) std::sscanf(e, "%d,%d", &bt, &t);
+    if (B <= 2) {   // one or two fits: the whole chip in lockstep
+        L.Bt = B;
+        L.T = chooseWorkgroups(K, B, device);
+    } else if (bt > 0 && t > 0) {
+        L.Bt = std::min(bt, 256);
+        L.T = std::min(t, res::MAXG);
+    } else {        // independent fits: one workgroup each, no grid exchange at all
+        L.Bt = 1;
+        L.T = 1;
     }
+    L.nTeams = (B + L.Bt - 1) / L.Bt;
+    return L;
 }
 
-// Run one persistent launch over fits [0, B) described by the host arrays; fills
-// betas/flags/iters/ll. kwHost: per-row weights of a single fit (or null).
+// Fits per persistent launch: lockstep teams must be co-resident, so a launch holds
+// what fits on the chip; one-workgroup teams are independent and take any count,
+// bounded by the per-team state (xb per team, per-pid arrays for grouped models)
+// staying within 16 GiB.
+static int chunkFits(const Layout& L, int64_t K, int64_t N, bool grouped, int realBytes) {
+    if (L.T > 1) return L.Bt * std::max(1, 2048 / (L.T * 4));
+    int BP = 1;
+    while (BP < L.Bt) BP *= 4;
+    const double perTeam = (double)K * BP * realBytes + (grouped ? (double)N * BP * 40.0 : 0.0);
+    const double budget = 16.0 * (1ull << 30);
+    const int teams = (int)std::max(1.0, std::min(4096.0, budget / perTeam));
+    return teams * L.Bt;
+}
+
+// Launch of kFindMode<Real, MODEL, BP> over nTeams teams of T workgroups; defined
+// per model in resident_<model>.hip so the instantiations compile in parallel.
+template <typename Real, int MODEL>
+void launchByBP(int BP, const res::Params<Real>* teams, int nTeams, int T, hipStream_t s);
+
+// Run fits [0, B) (host arrays indexed by fit) as persistent launches until all
+// have finished; fills betas/flags/iters/ll. kwHost: row weights of a single fit.
 template <typename Real>
 static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
                         const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
                         int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps,
-                        int& nLaunches) {
+                        int& nLaunches, Layout L) {
     auto& D = *E.D;
     hipStream_t s = E.s;
     const int64_t K = D.K;
@@ -207,10 +238,13 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     const bool grouped = (m == M_CLR || m == M_COX);
     if (!E.res) E.res = new ResWork<Real>();
     auto& W = *static_cast<ResWork<Real>*>(E.res);
-    const int G = chooseWorkgroups(K, B, D.device);
-    const int BP = padFits(B);
-    if (BP > 256) throw std::invalid_argument("at most 256 fits per batched launch");
-    // ---- structure for this G -------------------------------------------------
+    const int G = L.T;
+    const int nTeams = L.nTeams;
+    const int BP = padFits(L.Bt);
+    if (BP > 256) throw std::invalid_argument("at most 256 fits per team");
+    if ((int64_t)nTeams * L.Bt < B || (int64_t)(nTeams - 1) * L.Bt >= B)
+        throw std::logic_error("layout does not cover the fits");
+    // ---- structure for this team size -------------------------------------------
     if (W.G != G) {
         W.freeAll();
         W.G = G;
@@ -243,8 +277,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         for (int w = 0; w <= G; ++w) {
             int64_t r = (K * w) / G;
             if (grouped && w > 0 && w < G) {
-                int64_t p = (r < K) ? pid[r] : N;   // first pid starting at or after r
-                if (m == M_CLR) p = (r < K) ? D.strat[r] : N;
+                int64_t p = (m == M_CLR) ? ((r < K) ? D.strat[r] : N) : ((r < K) ? pid[r] : N);
                 if (p < N && pstart[p] < r) ++p;
                 r = (p < N) ? pstart[p] : K;
                 ps[w] = (int32_t)p;
@@ -280,18 +313,20 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     }
     const int64_t N = W.N;
     // ---- per-batch buffers ------------------------------------------------------
-    if (W.B != B || W.BP != BP) {
+    if (W.B != B || W.BP != BP || W.nTeams != nTeams) {
         W.freeBatch();
         W.B = B;
         W.BP = BP;
+        W.nTeams = nTeams;
         double& Bb = W.bytes;
-        W.xb = dalloc<Real>((size_t)K * BP, Bb);
+        W.xb = dalloc<Real>((size_t)nTeams * K * BP, Bb);
         if (grouped) {
-            W.denom = dalloc<double>((size_t)N * BP, Bb);
-            W.Dacc = dalloc<double>((size_t)N * BP, Bb);
-            W.R1 = dalloc<double>((size_t)N * BP, Bb);
-            W.R2 = dalloc<double>((size_t)N * BP, Bb);
-            W.nW = dalloc<double>((size_t)N * BP, Bb);
+            const size_t pn = (size_t)nTeams * N * BP;
+            W.denom = dalloc<double>(pn, Bb);
+            W.Dacc = dalloc<double>(pn, Bb);
+            W.R1 = dalloc<double>(pn, Bb);
+            W.R2 = dalloc<double>(pn, Bb);
+            W.nW = dalloc<double>(pn, Bb);
         }
         W.beta = dalloc<double>((size_t)B * J, Bb);
         W.hDelta = dalloc<double>((size_t)B * J, Bb);
@@ -312,14 +347,15 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         W.ctlFlag = dalloc<int>(B, Bb);
         W.ctlLast = dalloc<double>(B, Bb);
         W.ll = dalloc<double>(B, Bb);
-        W.epoch = dalloc<unsigned>(G, Bb);
-        W.slots = dalloc<double>((size_t)2 * G * BP * res::SLOTW, Bb);
-        W.bcast = dalloc<double>((size_t)2 * BP * 4, Bb);
+        W.epoch = dalloc<unsigned>((size_t)nTeams * G, Bb);
+        W.slots = dalloc<double>((size_t)nTeams * 2 * G * BP * res::SLOTW, Bb);
+        W.bcast = dalloc<double>((size_t)nTeams * 2 * BP * 4, Bb);
         W.err = dalloc<int>(1, Bb);
         W.allDone = dalloc<int>(1, Bb);
         W.steps = dalloc<unsigned long long>(1, Bb);
         W.stats = dalloc<double>(8, Bb);
         W.kw = dalloc<Real>(K, Bb);
+        W.teams = dalloc<res::Params<Real>>(nTeams, Bb);
     }
     // ---- inputs ---------------------------------------------------------------
     std::vector<unsigned char> excl(J, 0), fix(J, 0);
@@ -341,8 +377,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     h2d(W.held, hh.data(), B, s);
     if (rowFold) h2d(W.rowFold, rowFold, K, s);
     if (kwHost) h2dConvert(W.kw, kwHost, K, s);
-    // XjY: total (and per fold), or weighted for a single weighted fit
-    {
+    {   // XjY: total (and per fold), or weighted for a single weighted fit
         const int grid = std::min(J, 4096);
         if (kwHost) {
             kXjY<Real, true><<<grid, BLOCK, 0, s>>>(J, K, D.dFmt, D.dColPtr, D.dDensePtr, D.dRows, D.dVals, D.dDense,
@@ -369,39 +404,29 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
             ft[f] = kwHost ? totalW : total - ((rowFold && hh[f] >= 0) ? byFold[hh[f]] : 0.0);
     }
     h2d(W.fixedTerm, ft.data(), B, s);
-    // xb = X beta for every fit (column order, as CyclicCoordinateDescent::computeXBeta)
-    CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)K * BP * sizeof(Real), s));
-    {
-        std::vector<double> colBeta(B);
-        double* dColBeta = nullptr;
-        double tmpBytes = 0.0;
-        dColBeta = dalloc<double>((size_t)J * B, tmpBytes);
-        std::vector<double> all((size_t)J * B);
-        for (int j = 0; j < J; ++j)
-            for (int f = 0; f < B; ++f) all[(size_t)j * B + f] = betas[(size_t)f * J + j];
-        h2d(dColBeta, all.data(), (size_t)J * B, s);
-        for (int j = 0; j < J; ++j) {
-            bool any = false;
-            for (int f = 0; f < B; ++f) any |= all[(size_t)j * B + f] != 0.0;
-            if (!any) continue;
-            const Col<Real> c = E.col(j);
-            if (c.n == 0) continue;
-            CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, dColBeta + (size_t)j * B, B, BP, W.xb)));
-        }
-        CCD_HIP(hipGetLastError());
-        CCD_HIP(hipStreamSynchronize(s));
-        dfree(dColBeta);
+    // xb = X beta for every fit (column order, as CyclicCoordinateDescent::computeXBeta);
+    // team t's block is xb[t*K*BP ...], fit-minor inside the team
+    CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)nTeams * K * BP * sizeof(Real), s));
+    for (int j = 0; j < J; ++j) {
+        bool any = false;
+        for (int f = 0; f < B && !any; ++f) any = betas[(size_t)f * J + j] != 0.0;
+        const Col<Real> c = E.col(j);
+        if (!any || c.n == 0) continue;
+        CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, W.beta, J, j, B, L.Bt, BP, K, W.xb)));
     }
-    CCD_HIP(hipMemsetAsync(W.epoch, 0, G * sizeof(unsigned), s));
-    CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
+    CCD_HIP(hipGetLastError());
     CCD_HIP(hipMemsetAsync(W.steps, 0, sizeof(unsigned long long), s));
     CCD_HIP(hipMemsetAsync(W.stats, 0, 8 * sizeof(double), s));
     CCD_HIP(hipMemsetAsync(W.done, 0, B * sizeof(int), s));
     CCD_HIP(hipMemsetAsync(W.flag, 0, B * sizeof(int), s));
     CCD_HIP(hipMemsetAsync(W.iters, 0, B * sizeof(int), s));
-    // ---- parameters -----------------------------------------------------------
+    CCD_HIP(hipMemsetAsync(W.ctlIter, 0, B * sizeof(int), s));
+    // ---- parameters: one block per team -------------------------------------------
+    int rateKHz = 0;
+    CCD_HIP(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, D.device));
+    if (rateKHz <= 0) rateKHz = 100000;
     res::Params<Real> P{};
-    P.K = K; P.J = J; P.G = G; P.B = B; P.N = grouped ? N : K;
+    P.K = K; P.J = J; P.G = G; P.B = L.Bt; P.N = grouped ? N : K;
     P.fmt = D.dFmt; P.colPtr = D.dColPtr; P.densePtr = D.dDensePtr; P.rows = D.dRows; P.vals = D.dVals;
     P.dense = D.dDense; P.y = D.dY;
     P.seg = D.dStrat;
@@ -410,24 +435,32 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     P.rowStart = W.rowStart; P.pidStart = W.pidStart; P.colOff = W.colOff;
     P.kw = kwHost ? W.kw : nullptr;
     P.rowFold = rowFold ? W.rowFold : nullptr;
-    P.held = W.held;
-    P.xb = W.xb; P.denom = W.denom; P.Dacc = W.Dacc; P.R1 = W.R1; P.R2 = W.R2; P.nW = W.nW;
-    P.beta = W.beta; P.hDelta = W.hDelta; P.var = W.var; P.initBound = W.initBound;
     P.exclude = W.exclude; P.fixed = W.fixed;
     P.xjyTot = W.xjyTot; P.xjyFold = rowFold ? W.xjyFold : nullptr; P.F = F; P.xjyW = kwHost ? W.xjyW : nullptr;
-    P.fixedTermFit = W.fixedTerm;
     P.priorType = a.prior_type; P.convType = a.convergence_type; P.tol = a.tolerance; P.maxIt = a.max_iterations;
-    P.flag = W.flag; P.done = W.done; P.iters = W.iters; P.ll = W.ll;
-    P.ctlIter = W.ctlIter; P.ctlFlag = W.ctlFlag; P.ctlLast = W.ctlLast;
-    P.epoch = W.epoch; P.slots = W.slots; P.bcast = W.bcast; P.err = W.err;
-    int rateKHz = 0;
-    CCD_HIP(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, D.device));
-    if (rateKHz <= 0) rateKHz = 100000;
+    P.err = W.err; P.allDoneOut = W.allDone; P.steps = W.steps; P.stats = W.stats;
     P.deadlineTicks = (unsigned long long)rateKHz * 1000ull * 20ull;   // 20 s for one grid exchange
-    P.allRead = (B <= 2) ? 1 : 0;
-    P.steps = W.steps;
-    P.stats = W.stats;
-    P.allDoneOut = W.allDone;
+    P.allRead = (L.Bt <= 2 || G == 1) ? 1 : 0;
+    std::vector<res::Params<Real>> teamP(nTeams, P);
+    for (int t = 0; t < nTeams; ++t) {
+        auto& Q = teamP[t];
+        const size_t f0 = (size_t)t * L.Bt;
+        Q.B = std::min(L.Bt, B - (int)f0);
+        Q.xb = W.xb + (size_t)t * K * BP;
+        if (grouped) {
+            const size_t o = (size_t)t * N * BP;
+            Q.denom = W.denom + o; Q.Dacc = W.Dacc + o; Q.R1 = W.R1 + o; Q.R2 = W.R2 + o; Q.nW = W.nW + o;
+        }
+        Q.beta = W.beta + f0 * J; Q.hDelta = W.hDelta + f0 * J;
+        Q.var = W.var + f0; Q.initBound = W.initBound + f0; Q.held = W.held + f0;
+        Q.fixedTermFit = W.fixedTerm + f0;
+        Q.flag = W.flag + f0; Q.done = W.done + f0; Q.iters = W.iters + f0; Q.ll = W.ll + f0;
+        Q.ctlIter = W.ctlIter + f0; Q.ctlFlag = W.ctlFlag + f0; Q.ctlLast = W.ctlLast + f0;
+        Q.epoch = W.epoch + (size_t)t * G;
+        Q.slots = W.slots + (size_t)t * 2 * G * BP * res::SLOTW;
+        Q.bcast = W.bcast + (size_t)t * 2 * BP * 4;
+    }
+    h2d(W.teams, teamP.data(), nTeams, s);
     // ---- launches: each runs up to `budget` cycles, state stays in HBM -----------
     int budget = 32;
     if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) budget = std::max(1, std::atoi(e));
@@ -435,16 +468,20 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     const auto tWall = std::chrono::steady_clock::now();
     CCD_HIP(hipEventRecord(E.ev0, s));
     int done = 0, launches = 0;
+    const int one = 1;
     while (!done) {
-        CCD_HIP(hipMemsetAsync(W.epoch, 0, G * sizeof(unsigned), s));
+        CCD_HIP(hipMemsetAsync(W.epoch, 0, (size_t)nTeams * G * sizeof(unsigned), s));
         CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
-        P.resume = launches > 0 ? 1 : 0;
-        P.cycleBudget = budget;
+        h2d(W.allDone, &one, 1, s);
+        if (launches <= 1) {   // the first launch starts the fits, later ones resume them
+            for (auto& Q : teamP) { Q.resume = launches; Q.cycleBudget = budget; }
+            h2d(W.teams, teamP.data(), nTeams, s);
+        }
         switch (m) {
-        case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, P, s); break;
-        case M_POISSON: launchByBP<Real, M_POISSON>(BP, P, s); break;
-        case M_CLR: launchByBP<Real, M_CLR>(BP, P, s); break;
-        default: launchByBP<Real, M_COX>(BP, P, s); break;
+        case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, W.teams, nTeams, G, s); break;
+        case M_POISSON: launchByBP<Real, M_POISSON>(BP, W.teams, nTeams, G, s); break;
+        case M_CLR: launchByBP<Real, M_CLR>(BP, W.teams, nTeams, G, s); break;
+        default: launchByBP<Real, M_COX>(BP, W.teams, nTeams, G, s); break;
         }
         CCD_HIP(hipGetLastError());
         int hs[2] = {0, 0};
@@ -454,13 +491,10 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         if (hs[0]) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
         done = hs[1];
         ++launches;
-        if (progress) {
-            std::vector<int> it(B);
-            CCD_HIP(hipMemcpy(it.data(), W.ctlIter, B * sizeof(int), hipMemcpyDeviceToHost));
-            std::fprintf(stderr, "[ccd] launch %d: %d cycles done, %s (%.1fs)\n", launches, launches * budget,
-                         done ? "all fits finished" : "fits still running",
+        if (progress)
+            std::fprintf(stderr, "[ccd] launch %d (%d teams x %d workgroups, %d fits): %d cycles, %s (%.1fs)\n",
+                         launches, nTeams, G, B, launches * budget, done ? "all fits finished" : "fits still running",
                          std::chrono::duration<double>(std::chrono::steady_clock::now() - tWall).count());
-        }
     }
     CCD_HIP(hipEventRecord(E.ev1, s));
     CCD_HIP(hipStreamSynchronize(s));
@@ -479,8 +513,8 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     CCD_HIP(hipMemcpy(st8, W.stats, sizeof(st8), hipMemcpyDeviceToHost));
     algoBytes = st8[0];
     if (std::getenv("CCD_PHASES"))
-        std::fprintf(stderr, "[ccd] G=%d B=%d BP=%d steps=%llu  ms: column pass %.1f  exchange %.1f  update %.1f  objective %.1f  (kernel %.1f)\n",
-                     G, B, BP, st, st8[1] / rateKHz, st8[2] / rateKHz, st8[3] / rateKHz, st8[4] / rateKHz, (double)ms);
+        std::fprintf(stderr, "[ccd] teams=%d T=%d B=%d BP=%d steps=%llu  ms (summed over teams): column pass %.1f  exchange %.1f  update %.1f  objective %.1f  (kernel %.1f)\n",
+                     nTeams, G, B, BP, st, st8[1] / rateKHz, st8[2] / rateKHz, st8[3] / rateKHz, st8[4] / rateKHz, (double)ms);
 }
 
 template <typename Real>
@@ -498,7 +532,7 @@ void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_res
         double ms = 0, bytes = 0, steps = 0;
         int nl = 0;
         runResident<Real>(*this, a, 1, &var, nullptr, nullptr, 0, a.weights, &initBound, b.data(), &flag, &it, &ll,
-                          ms, bytes, steps, nl);
+                          ms, bytes, steps, nl, chooseLayout(D->K, 1, D->device));
         ++count;
         if (timing) {
             counters.deviceMs += ms;
@@ -551,11 +585,15 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     std::vector<double> lls(n, 0.0), bound(n, a.initial_bound);
     std::vector<int> pending(n);
     std::iota(pending.begin(), pending.end(), 0);
+    const bool grouped = D->model == M_CLR || D->model == M_COX;
+    const int64_t nGroups = D->model == M_CLR ? D->nStrata : D->K;
     while (!pending.empty()) {
-        // chunks of at most 256 fits per launch
         std::vector<int> retry;
-        for (size_t c0 = 0; c0 < pending.size(); c0 += 256) {
-            const int B = (int)std::min<size_t>(256, pending.size() - c0);
+        const Layout L0 = chooseLayout(D->K, (int)pending.size(), D->device);
+        const size_t chunk = (size_t)chunkFits(L0, D->K, nGroups, grouped, (int)sizeof(Real));
+        for (size_t c0 = 0; c0 < pending.size(); c0 += chunk) {
+            const int B = (int)std::min(chunk, pending.size() - c0);
+            const Layout L = chooseLayout(D->K, B, D->device);
             std::vector<double> v(B), bd(B), bb((size_t)B * J), l(B);
             std::vector<int32_t> h(B);
             std::vector<int> fl(B), it(B);
@@ -569,7 +607,7 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
             double ms = 0, bytes = 0, steps = 0;
             int nl = 0;
             runResident<Real>(*this, a, B, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(), bb.data(),
-                              fl.data(), it.data(), l.data(), ms, bytes, steps, nl);
+                              fl.data(), it.data(), l.data(), ms, bytes, steps, nl, L);
             if (timing) {
                 counters.deviceMs += ms;
                 counters.launches += nl;

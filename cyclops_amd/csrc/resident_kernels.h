@@ -15,6 +15,7 @@
 #pragma once
 
 #include "kernels.h"
+#include "../../include/cyclops_amd.h"
 
 namespace ccd {
 namespace res {
@@ -92,7 +93,8 @@ struct Params {
     double* ctlLast;      // [B]
     int resume;           // continue a run started by an earlier launch
     int cycleBudget;      // cycles this launch may run
-    int* allDoneOut;      // set when every fit has finished
+    int* allDoneOut;      // cleared by any team with a fit still running
+    int w;                // workgroup index inside its team (set in the kernel)
     int* iters;           // [B]
     double* ll;           // [B]
     // sync / scratch
@@ -113,8 +115,10 @@ __device__ __forceinline__ unsigned long long now() { return wall_clock64(); }
 
 // Cross-workgroup words (epochs, slots, broadcasts, beta and bound state) are read
 // and written only with relaxed agent-scope atomics, which are coherent across the
-// XCDs per word without flushing or invalidating whole L2 caches; workgroup-scope
-// fences order a workgroup's own accesses around the epoch store and load.
+// XCDs per word without flushing or invalidating whole L2 caches. Before a
+// workgroup publishes an epoch, every wave waits for its stores to be acknowledged
+// (drainStores), so a reader that sees the epoch sees the data; readers issue their
+// data loads only after the epoch load has returned.
 // Per-row and per-pid state is touched only by the workgroup that owns it.
 __device__ __forceinline__ void ast(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -123,10 +127,11 @@ __device__ __forceinline__ double ald(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void publish(unsigned* epoch, unsigned seq) {
+__device__ __forceinline__ void publish(unsigned* epoch, unsigned seq, int w) {
+    drainStores();   // this workgroup's slot/broadcast/beta stores land before the epoch
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&epoch[blockIdx.x], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_store(&epoch[w], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wait until every workgroup has published `seq`. Returns false on timeout or a
@@ -528,7 +533,13 @@ struct Shared {
 template <typename Real, int BP, class Fn>
 __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
                                            Shared<BP>& s, Fn fn, bool ordered = true) {
-    publish(P.epoch, ++seq);
+    if (P.G == 1) {   // a one-workgroup team reduces in LDS only
+        if (threadIdx.x == 0)
+            for (int f = 0; f < P.B; ++f) fn(f, s.mon[f], s.out + 4 * f);
+        __syncthreads();
+        return true;
+    }
+    publish(P.epoch, ++seq, P.w);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     const double* slot = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
     auto comb = [](const Mon& a, const Mon& b) { return monCombine(a, b); };
@@ -558,7 +569,7 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
         __syncthreads();
     } else {
         double* bc = P.bcast + (size_t)((seq + 1) & 1) * BP * 4;
-        for (int f = blockIdx.x; f < P.B; f += P.G) {
+        for (int f = P.w; f < P.B; f += P.G) {
             const Mon m = reduceOne(f);
             if (threadIdx.x == 0) {
                 double o[4] = {0, 0, 0, 0};
@@ -566,7 +577,7 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
                 for (int i = 0; i < 4; ++i) ast(&bc[f * 4 + i], o[i]);
             }
         }
-        publish(P.epoch, ++seq);
+        publish(P.epoch, ++seq, P.w);
         if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
         for (int i = threadIdx.x; i < P.B * 4; i += BLOCK) s.out[i] = ald(&bc[i]);
         __syncthreads();
@@ -578,9 +589,10 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
 template <typename Real, int BP>
 __device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, const Shared<BP>& s,
                                            bool ordered = true) {
+    if (P.G == 1) return;
     double* slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
     for (int f = threadIdx.x; f < BP; f += BLOCK) {
-        double* q = slot + ((size_t)blockIdx.x * BP + f) * SLOTW;
+        double* q = slot + ((size_t)P.w * BP + f) * SLOTW;
         const Mon& m = s.mon[f];
         ast(q, m.g);
         ast(q + 1, m.h);
@@ -594,7 +606,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
                                           Shared<BP>& s) {
     using GE = Geo<BP>;
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
-    const int w = blockIdx.x;
+    const int w = P.w;
     const int64_t q0 = P.pidStart[w], q1 = P.pidStart[w + 1];
     const int64_t c = (q1 - q0 + GE::NSG - 1) / GE::NSG;
     int64_t p0 = q0 + sg * c, p1 = p0 + c;
@@ -624,7 +636,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
         double* o = slot + ((size_t)w * BP + f) * SLOTW;
         ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
     }
-    publish(P.epoch, ++seq);
+    publish(P.epoch, ++seq, P.w);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     {
         const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
@@ -673,7 +685,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
         double* o = slot + ((size_t)w * BP + f) * SLOTW;
         ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
     }
-    publish(P.epoch, ++seq);
+    publish(P.epoch, ++seq, P.w);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     {
         const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
@@ -715,7 +727,7 @@ __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, 
     using GE = Geo<BP>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
-    const int w = blockIdx.x;
+    const int w = P.w;
     double acc[GE::FPL];
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
@@ -778,7 +790,7 @@ template <typename Real, int MODEL, int BP>
 __device__ __forceinline__ void initPids(const Params<Real>& P) {
     using GE = Geo<BP>;
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
-    const int w = blockIdx.x;
+    const int w = P.w;
     const int64_t a = P.pidStart[w], b = P.pidStart[w + 1];
     const int64_t c = (b - a + GE::NSG - 1) / GE::NSG;
     int64_t n0 = a + sg * c, n1 = n0 + c;
@@ -812,12 +824,16 @@ __device__ __forceinline__ void initPids(const Params<Real>& P) {
 }
 
 // The persistent findMode kernel (CyclicCoordinateDescent.cpp:909-1216 for every fit).
+// Launch = teams of T workgroups; team t runs the fits described by teams[t]
+// (its own data and state pointers), exchanging only inside the team.
 template <typename Real, int MODEL, int BP>
-__global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
+__global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restrict__ teams, int T) {
     using GE = Geo<BP>;
+    Params<Real> P = teams[blockIdx.x / T];
+    P.w = (int)(blockIdx.x % T);
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     __shared__ Shared<BP> s;
-    const int w = blockIdx.x;
+    const int w = P.w;
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
     const unsigned long long deadline = P.deadlineTicks;   // per-wait limit
     unsigned seq = 0;
@@ -872,6 +888,8 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
             return;
     }
 
+    double loc[4] = {0.0, 0.0, 0.0, 0.0};   // counters of workgroup 0, thread 0
+    unsigned long long locSteps = 0;
     bool allDone = false;
     {
         int act = 0;
@@ -975,10 +993,10 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
             }
             if (w == 0 && threadIdx.x == 0) {
                 const unsigned long long tD = now();
-                P.stats[1] += (double)(tB - tA);
-                P.stats[2] += (double)(tC - tB);
-                P.stats[3] += (double)(tD - tC);
-                atomicAdd(P.steps, 1ull);
+                loc[1] += (double)(tB - tA);
+                loc[2] += (double)(tC - tB);
+                loc[3] += (double)(tD - tC);
+                locSteps += 1;
                 // compulsory bytes of this coordinate (my algorithm): the column pass
                 // reads ids/values/weights once and xb of every active fit; the update
                 // reads and writes xb of every fit that moved
@@ -996,7 +1014,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
                     b += base + n * nUpd * 2.0 * sizeof(Real) + (GROUPED ? n * (4.0 + 16.0 * nUpd) : 0.0);
                     if (MODEL == M_COX) b += (double)P.N * P.B * 56.0;
                 }
-                P.stats[0] += b;
+                loc[0] += b;
             }
         }
         // convergence test of every fit (performCheckConvergence :839-906)
@@ -1026,12 +1044,18 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
             notDone |= act;
         }
         allDone = !__syncthreads_or(notDone);
-        if (w == 0 && threadIdx.x == 0) P.stats[4] += (double)(now() - tO);
+        if (w == 0 && threadIdx.x == 0) {   // flush this cycle's counters (once per cycle, not per coordinate)
+            atomicAdd(&P.stats[4], (double)(now() - tO));
+            for (int i = 0; i < 4; ++i) atomicAdd(&P.stats[i], loc[i]);
+            atomicAdd(P.steps, locSteps);
+            loc[0] = loc[1] = loc[2] = loc[3] = 0.0;
+            locSteps = 0;
+        }
         if (w == 0 && threadIdx.x == 0) {   // objective pass: y, xb of every fit, per-pid terms
             double nAct = 0;
             for (int f = 0; f < P.B; ++f) nAct += 1;
-            P.stats[0] += (double)P.K * (sizeof(Real) * nAct + sizeof(Real) + (P.rowFold ? 4.0 : 0.0)) +
-                          (GROUPED ? (double)P.N * nAct * 16.0 : 0.0);
+            atomicAdd(&P.stats[0], (double)P.K * (sizeof(Real) * nAct + sizeof(Real) + (P.rowFold ? 4.0 : 0.0)) +
+                                       (GROUPED ? (double)P.N * nAct * 16.0 : 0.0));
         }
     }
     if (own && w == 0) {   // persist the control state for a later launch
@@ -1042,7 +1066,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(Params<Real> P) {
             P.done[f] = s.done[f];
         }
     }
-    if (w == 0 && threadIdx.x == 0) *P.allDoneOut = allDone ? 1 : 0;
+    if (w == 0 && threadIdx.x == 0 && !allDone) atomicAnd(P.allDoneOut, 0);
     if (!allDone) return;
     // final log-likelihood of every fit (reported beside beta)
     objectivePartial<Real, MODEL, BP>(P, OBJ_LOGLIK, s);

@@ -1,0 +1,4 @@
+// Persistent findMode kernel instantiations for one model.
+#include "resident_launch.h"
+
+CCD_INSTANTIATE_LAUNCH(M_POISSON)

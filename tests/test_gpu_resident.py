@@ -118,6 +118,58 @@ def test_batched_fits_match_independent_oracle_fits(E, model, n_var):
         assert abs(res[i]["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
 
 
+@pytest.mark.parametrize("layout", ["1,1", "4,1", "8,4", "32,16"])
+@pytest.mark.parametrize("model", MODELS)
+def test_batch_layouts_agree(E, model, layout, monkeypatch):
+    """Every team shape (fits per team, workgroups per team) gives the oracle's fits:
+    one-workgroup teams of 1 or 4 fits, three co-resident 4-workgroup teams (the last
+    one partly filled) and one 16-workgroup lockstep team."""
+    d = _cases.ALL[model]()
+    folds = 3
+    rf = _fold_setup(d, model, folds)
+    var_list, held = [], []
+    for v in np.exp(np.linspace(np.log(0.05), np.log(2.0), 7)):
+        for f in range(folds):
+            var_list.append(v)
+            held.append(f)
+    print(f"[case] layout {layout} {model} n_fits={len(var_list)}", flush=True)
+    ptype = "normal" if model == "pr" else "laplace"
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    monkeypatch.setenv("CCD_BATCH_LAYOUT", layout)
+    eng = E.Engine(d, 64)
+    betas, res = eng.fit_batch(var_list, ptype, ctl, held_out_fold=held, row_fold=rf)
+    eng.close()
+    for i in [0, 5, 11, 20]:
+        w = (rf != held[i]).astype(float)
+        bo, ro = oracle.fit(d, create_prior(ptype, var_list[i]), ctl, weights=w)
+        assert res[i]["iterations"] == ro["iterations"], (i, res[i], ro)
+        assert _rel(betas[i], bo) <= 1e-6, (i, betas[i], bo)
+        assert abs(res[i]["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
+
+
+def test_co_resident_teams_are_exact_at_scale(E, monkeypatch):
+    """Several multi-workgroup teams sharing the chip exchange partial sums while
+    their workgroups drift apart in time; their fits must equal the fits of
+    one-workgroup teams (no exchange at all) to the last bit (FP32 bulk, FP64 sums of
+    FP32 terms are exact in any order here). Poisson workload at 0.2 scale: 200,000
+    rows, 4,001 columns, 64 fits, 3 cycles."""
+    from cyclops_amd import data as D
+    d, _ = D.make_config(1, scale=0.2)
+    rf = E.assign_folds(d.K, None, 4, _abi.SELECT_BY_ROW, 1001)
+    var_list = [float(v) for v in np.exp(np.linspace(np.log(0.1), np.log(10.0), 16)) for _ in range(4)]
+    held = [f for _ in range(16) for f in range(4)]
+    ctl = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=3)
+    eng = E.Engine(d, _abi.FP32)
+    out = {}
+    for layout in ["1,1", "16,64", "4,32", "1,16", "64,128"]:
+        print(f"[case] co-resident teams layout {layout}", flush=True)
+        monkeypatch.setenv("CCD_BATCH_LAYOUT", layout)
+        out[layout], _ = eng.fit_batch(var_list, "normal", ctl, held_out_fold=held, row_fold=rf)
+    eng.close()
+    for layout, b in out.items():
+        assert np.max(np.abs(b - out["1,1"])) <= 1e-12, layout
+
+
 @pytest.mark.parametrize("model", ["lr", "clr"])
 def test_cross_validation_matches_oracle(E, model):
     d = _cases.ALL[model]()
