@@ -34,6 +34,26 @@ __global__ __launch_bounds__(BLOCK) void kColOff(int J, int G, const int* __rest
     }
 }
 
+// Absolute entry range of workgroup w in column j: [rng[2(jG+w)], rng[2(jG+w)+1]).
+__global__ __launch_bounds__(BLOCK) void kColRange(int J, int G, const int* __restrict__ fmt,
+                                                  const int64_t* __restrict__ colPtr, const int* __restrict__ colOff,
+                                                  const int* __restrict__ rowStart, long long* __restrict__ rng) {
+    const int64_t total = (int64_t)J * G;
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < total; t += stride) {
+        const int j = (int)(t / G), w = (int)(t % G);
+        const int f = fmt[j];
+        if (f == F_INDICATOR || f == F_SPARSE) {
+            const int* off = colOff + (size_t)j * (G + 1);
+            rng[2 * t] = colPtr[j] + off[w];
+            rng[2 * t + 1] = colPtr[j] + off[w + 1];
+        } else {
+            rng[2 * t] = rowStart[w];
+            rng[2 * t + 1] = rowStart[w + 1];
+        }
+    }
+}
+
 // sum over column j of x*y, in total and per fold (F <= 32 folds).
 template <typename Real>
 __global__ __launch_bounds__(BLOCK) void kXjYFold(int J, int64_t K, int F, const int* __restrict__ fmt,
@@ -110,6 +130,8 @@ struct ResWork {
     int* rowStart = nullptr;
     int* pidStart = nullptr;
     int* colOff = nullptr;
+    long long* rng = nullptr;
+    long long* info = nullptr;
     int* pid = nullptr;       // Cox: unweighted tie pids
     int* pstart = nullptr;
     int* pseg = nullptr;
@@ -137,12 +159,12 @@ struct ResWork {
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
-        dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw); dfree(teams);
+        dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw); dfree(teams); dfree(info);
         B = BP = nTeams = 0;
     }
     void freeAll() {
         freeBatch();
-        dfree(rowStart); dfree(pidStart); dfree(colOff); dfree(pid); dfree(pstart); dfree(pseg);
+        dfree(rowStart); dfree(pidStart); dfree(colOff); dfree(rng); dfree(pid); dfree(pstart); dfree(pseg);
         G = 0;
     }
 };
@@ -298,6 +320,9 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         W.colOff = dalloc<int>((size_t)J * (G + 1), Bb);
         kColOff<<<gridFor((int64_t)J * (G + 1)), BLOCK, 0, s>>>(J, G, D.dFmt, D.dColPtr, D.dRows, W.rowStart, W.colOff);
         CCD_HIP(hipGetLastError());
+        W.rng = dalloc<long long>((size_t)J * G * 2, Bb);
+        kColRange<<<gridFor((int64_t)J * G), BLOCK, 0, s>>>(J, G, D.dFmt, D.dColPtr, W.colOff, W.rowStart, W.rng);
+        CCD_HIP(hipGetLastError());
         if (m == M_COX) {
             W.pid = dalloc<int>(K, Bb);
             h2d(W.pid, pid.data(), K, s);
@@ -332,6 +357,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         W.hDelta = dalloc<double>((size_t)B * J, Bb);
         W.var = dalloc<double>(B, Bb);
         W.initBound = dalloc<double>(B, Bb);
+        W.info = dalloc<long long>((size_t)J * 4, Bb);
         W.exclude = dalloc<unsigned char>(J, Bb);
         W.fixed = dalloc<unsigned char>(J, Bb);
         W.xjyTot = dalloc<double>(J, Bb);
@@ -372,6 +398,22 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     h2d(W.initBound, initBound, B, s);
     h2d(W.exclude, excl.data(), J, s);
     h2d(W.fixed, fix.data(), J, s);
+    // column table: fmt | exclude << 32, nnz | next non-fixed column << 32, c0
+    int firstJ = 0;
+    while (firstJ < J && fix[firstJ]) ++firstJ;
+    std::vector<long long> info((size_t)J * 4, 0);
+    {
+        int nextFree = firstJ;   // next non-fixed column after j, wrapping to firstJ
+        for (int j = J - 1; j >= 0; --j) {
+            const bool sp = D.fmt[j] == F_INDICATOR || D.fmt[j] == F_SPARSE;
+            const int64_t nnz = sp ? D.colPtr[j + 1] - D.colPtr[j] : K;
+            info[(size_t)j * 4 + 0] = (long long)(unsigned)D.fmt[j] | ((long long)excl[j] << 32);
+            info[(size_t)j * 4 + 1] = (long long)(unsigned)nnz | ((long long)nextFree << 32);
+            info[(size_t)j * 4 + 2] = sp ? D.colPtr[j] : 0;
+            if (!fix[j]) nextFree = j;
+        }
+    }
+    h2d(W.info, info.data(), (size_t)J * 4, s);
     std::vector<int32_t> hh(B, -1);
     if (held) std::copy(held, held + B, hh.begin());
     h2d(W.held, hh.data(), B, s);
@@ -433,6 +475,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     P.pid = (m == M_COX) ? W.pid : D.dStrat;
     P.pstart = W.pstart; P.pseg = W.pseg;
     P.rowStart = W.rowStart; P.pidStart = W.pidStart; P.colOff = W.colOff;
+    P.info = W.info; P.rng = W.rng; P.firstJ = firstJ;
     P.kw = kwHost ? W.kw : nullptr;
     P.rowFold = rowFold ? W.rowFold : nullptr;
     P.exclude = W.exclude; P.fixed = W.fixed;
@@ -440,7 +483,10 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     P.priorType = a.prior_type; P.convType = a.convergence_type; P.tol = a.tolerance; P.maxIt = a.max_iterations;
     P.err = W.err; P.allDoneOut = W.allDone; P.steps = W.steps; P.stats = W.stats;
     P.deadlineTicks = (unsigned long long)rateKHz * 1000ull * 20ull;   // 20 s for one grid exchange
-    P.allRead = (L.Bt <= 2 || G == 1) ? 1 : 0;
+    // all-read: every workgroup reduces and steps every fit after one exchange; used
+    // when the partials to read per coordinate are few (G x fits), else the owner of
+    // each fit steps it and broadcasts the delta (a second exchange)
+    P.allRead = (L.Bt <= 2 || G == 1 || (!grouped && (int64_t)G * BP <= 1024)) ? 1 : 0;
     std::vector<res::Params<Real>> teamP(nTeams, P);
     for (int t = 0; t < nTeams; ++t) {
         auto& Q = teamP[t];

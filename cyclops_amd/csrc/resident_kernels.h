@@ -59,6 +59,9 @@ struct Params {
     const int* rowStart;  // [G+1]
     const int* pidStart;  // [G+1]
     const int* colOff;    // [J*(G+1)] entry offset of workgroup w inside column j (INDICATOR/SPARSE)
+    const long long* info;   // [J][4] per column: fmt | flags << 32, nnz | next << 32, c0, 0
+    const long long* rng;    // [J][G][2] absolute entry range of workgroup w in column j
+    int firstJ;           // first column that is not fixed (J if none)
     // weights
     const Real* kw;       // [K] (single fit) or null
     const int* rowFold;   // [K] or null
@@ -134,27 +137,29 @@ __device__ __forceinline__ void publish(unsigned* epoch, unsigned seq, int w) {
     if (threadIdx.x == 0) __hip_atomic_store(&epoch[w], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wait until every workgroup has published `seq`. Returns false on timeout or a
-// raised error flag (every waiter then leaves the kernel).
+// Wait until every workgroup has published `seq`. Lane w watches epoch[w] and
+// stops loading it once it has arrived, so a round of polling costs one load per
+// workgroup still missing rather than one per team member. Returns false on
+// timeout or a raised error flag (every waiter then leaves the kernel).
 __device__ __forceinline__ bool waitAll(const unsigned* epoch, unsigned seq, int G, int* err,
                                         unsigned long long ticks) {
     const unsigned long long deadline = now() + ticks;
-    for (;;) {
-        int ok = 1;
-        for (int w = threadIdx.x; w < G; w += BLOCK)
-            ok &= (__hip_atomic_load(&epoch[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= seq) ? 1 : 0;
-        if (__syncthreads_and(ok)) break;
-        int bad = 0;
-        if (threadIdx.x == 0) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) bad = 1;
-            else if (now() > deadline) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                bad = 1;
+    int bad = 0;
+    for (int w = threadIdx.x; w < G && !bad; w += BLOCK) {
+        unsigned spins = 0;
+        while (__hip_atomic_load(&epoch[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < seq) {
+            if ((++spins & 63u) == 0) {   // another workgroup's timeout ends this wait too
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { bad = 1; break; }
+                if (now() > deadline) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bad = 1;
+                    break;
+                }
             }
+            __builtin_amdgcn_s_sleep(1);
         }
-        if (__syncthreads_or(bad)) return false;
-        __builtin_amdgcn_s_sleep(1);
     }
+    if (__syncthreads_or(bad)) return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return true;
 }
@@ -233,15 +238,122 @@ struct Unroll {
     static constexpr int U = Geo<BP>::FPL >= 4 ? 4 : (BP >= 16 ? 16 : 8);
 };
 
-template <typename Real, int MODEL, int BP, int FMT>
-__device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t a, int64_t b, int64_t c0,
-                                         Mon (&acc)[Geo<BP>::FPL], const int* active) {
+// U consecutive entries of one sub-group's chunk with everything the column pass
+// and the update need; held in registers across the grid exchange when a chunk
+// has at most U entries (the update then only stores).
+template <typename Real, int MODEL, int BP, int U = Unroll<BP>::U>
+struct Batch {
     using GE = Geo<BP>;
+    int kk[U];
+    Real xx[U];
+    Real xv[U][GE::FPL];
+    Real ww[U][GE::FPL];
+    int sgm[U], pp[U];
+    double r1[U][GE::FPL], r2[U][GE::FPL];
+    bool val[U];
+};
+
+// Row ids and values of entries [e, e + U) of the chunk ending at n1.
+template <typename Real, int MODEL, int BP, int FMT, int U>
+__device__ __forceinline__ void loadRows(const Params<Real>& P, int j, int64_t e, int64_t n1,
+                                         Batch<Real, MODEL, BP, U>& bt) {
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        bt.val[u] = e + u < n1;
+        bt.kk[u] = bt.val[u] ? (int)rowAt<Real, FMT>(P, e + u) : 0;
+        bt.xx[u] = bt.val[u] ? xAt<Real, FMT>(P, j, e + u) : Real(0);
+        if constexpr (GROUPED) {
+            bt.sgm[u] = bt.val[u] ? P.seg[bt.kk[u]] : 0;
+            bt.pp[u] = bt.val[u] ? P.pid[bt.kk[u]] : 0;
+        }
+    }
+}
+
+// Per-fit state of the rows of a batch (xb, weights, risk-set ratios).
+template <typename Real, int MODEL, int BP, int U>
+__device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on)[Geo<BP>::FPL],
+                                          Batch<Real, MODEL, BP, U>& bt) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    const int sl = threadIdx.x % GE::SUB;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            const int f = sl + i * GE::SUB;
+            const bool go = bt.val[u] && on[i];
+            bt.xv[u][i] = go ? P.xb[(int64_t)bt.kk[u] * BP + f] : Real(0);
+            if constexpr (MODEL != M_CLR) bt.ww[u][i] = go ? weightOf(P, bt.kk[u], f) : Real(0);
+            if constexpr (GROUPED) {
+                const size_t q = (size_t)bt.pp[u] * BP + f;
+                bt.r1[u][i] = go ? P.R1[q] : 0.0;
+                bt.r2[u][i] = go ? P.R2[q] : 0.0;
+            }
+        }
+}
+
+template <typename Real, int MODEL, int BP, int FMT, int U>
+__device__ __forceinline__ void loadBatch(const Params<Real>& P, int j, int64_t e, int64_t n1,
+                                          const bool (&on)[Geo<BP>::FPL], Batch<Real, MODEL, BP, U>& bt) {
+    loadRows<Real, MODEL, BP, FMT, U>(P, j, e, n1, bt);
+    loadState<Real, MODEL, BP, U>(P, on, bt);
+}
+
+// Gradient terms of a batch, strictly in entry order.
+template <typename Real, int MODEL, int BP, int U>
+__device__ __forceinline__ void accumBatch(const Batch<Real, MODEL, BP, U>& bt, const bool (&on)[Geo<BP>::FPL],
+                                           Mon (&acc)[Geo<BP>::FPL], int& prevSeg, bool& hasPrev) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!bt.val[u]) break;
+        const Real x = bt.xx[u];
+        bool reset = false;
+        if constexpr (GROUPED) {
+            reset = hasPrev && bt.sgm[u] != prevSeg;
+            prevSeg = bt.sgm[u];
+            hasPrev = true;
+        }
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            if (!on[i]) continue;
+            const Real ee = dexp(bt.xv[u][i]);
+            if constexpr (MODEL == M_LOGISTIC) {
+                const Real wt = bt.ww[u][i];
+                const Real pr = ee / (Real(1) + ee);
+                acc[i].g += (double)(wt * pr * x);
+                acc[i].h += (double)(wt * x * x * pr * (Real(1) - pr));
+            } else if constexpr (MODEL == M_POISSON) {
+                const Real wt = bt.ww[u][i];
+                acc[i].g += (double)(wt * ee * x);
+                acc[i].h += (double)(wt * ee * x * x);
+            } else {
+                Real uu = ee * x;
+                if constexpr (MODEL == M_COX) uu *= bt.ww[u][i];
+                const double uv = (double)uu;
+                if (reset) {
+                    acc[i].t = 0.0;
+                    acc[i].r = 1.0;
+                }
+                const double sv = acc[i].t + uv;
+                acc[i].l += bt.r2[u][i] * uv * (2.0 * sv - uv);
+                if (acc[i].r == 0.0) acc[i].m += bt.r2[u][i] * uv;
+                acc[i].t = sv;
+                acc[i].g += uv * bt.r1[u][i];
+                acc[i].h += uv * (double)x * bt.r1[u][i];
+            }
+        }
+    }
+}
+
+// Column pass over the sub-group chunk [n0, n1) (run-aligned for grouped models).
+template <typename Real, int MODEL, int BP, int FMT>
+__device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n0, int64_t n1, int64_t c0,
+                                         Mon (&acc)[Geo<BP>::FPL], const bool (&on)[Geo<BP>::FPL]) {
     constexpr int U = Unroll<BP>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
-    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
-    int64_t n0, n1;
-    subChunk<Real, FMT, GROUPED>(P, a, b, sg, GE::NSG, n0, n1);
     int prevSeg = 0;
     bool hasPrev = false;
     if constexpr (GROUPED) {
@@ -250,79 +362,58 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t a
             hasPrev = true;
         }
     }
-    bool on[GE::FPL];
-#pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) {
-        const int f = sl + i * GE::SUB;
-        on[i] = f < P.B && active[f];
-    }
     for (int64_t e = n0; e < n1; e += U) {
-        int64_t kk[U];
-        Real xx[U], ww[U][GE::FPL], xv[U][GE::FPL];
-        int sgm[U], pp[U];
-        double r1[U][GE::FPL], r2[U][GE::FPL];
-        bool val[U];
+        Batch<Real, MODEL, BP, U> bt;
+        loadBatch<Real, MODEL, BP, FMT, U>(P, j, e, n1, on, bt);
+        accumBatch<Real, MODEL, BP, U>(bt, on, acc, prevSeg, hasPrev);
+    }
+}
+
+// Update of a chunk that fitted in one batch: xb from the registers of the column
+// pass (fits that moved were active there), per-run denominators as updatePass.
+template <typename Real, int MODEL, int BP, int U>
+__device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Real, MODEL, BP, U>& bt,
+                                           const double (&dl)[Geo<BP>::FPL]) {
+    using GE = Geo<BP>;
+    constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    const int sl = threadIdx.x % GE::SUB;
+    double acc[GE::FPL];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            val[u] = e + u < n1;
-            kk[u] = val[u] ? rowAt<Real, FMT>(P, e + u) : 0;
-            xx[u] = val[u] ? xAt<Real, FMT>(P, j, e + u) : Real(0);
+    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!bt.val[u]) break;
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            if (dl[i] == 0.0) continue;
+            const int f = sl + i * GE::SUB;
+            const Real old = bt.xv[u][i];
+            const Real nv = old + (Real)dl[i] * bt.xx[u];
+            P.xb[(int64_t)bt.kk[u] * BP + f] = nv;
             if constexpr (GROUPED) {
-                sgm[u] = val[u] ? P.seg[kk[u]] : 0;
-                pp[u] = val[u] ? P.pid[kk[u]] : 0;
+                Real diff = dexp(nv) - dexp(old);
+                if constexpr (MODEL == M_COX) diff *= bt.ww[u][i];
+                acc[i] += (double)diff;
             }
         }
+        if constexpr (GROUPED) {
+            const int pn = (u + 1 < U && bt.val[u + 1]) ? bt.pp[u + 1] : -1;
+            if (pn != bt.pp[u]) {   // last entry of a run of equal pid
+                const int p = bt.pp[u];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int i = 0; i < GE::FPL; ++i) {
-                const int f = sl + i * GE::SUB;
-                const bool go = val[u] && on[i];
-                xv[u][i] = go ? P.xb[kk[u] * BP + f] : Real(0);
-                if constexpr (MODEL != M_CLR) ww[u][i] = go ? weightOf(P, kk[u], f) : Real(0);
-                if constexpr (GROUPED) {
-                    const size_t q = (size_t)pp[u] * BP + f;
-                    r1[u][i] = go ? P.R1[q] : 0.0;
-                    r2[u][i] = go ? P.R2[q] : 0.0;
-                }
-            }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (!val[u]) break;
-            const Real x = xx[u];
-            bool reset = false;
-            if constexpr (GROUPED) {
-                reset = hasPrev && sgm[u] != prevSeg;
-                prevSeg = sgm[u];
-                hasPrev = true;
-            }
-#pragma unroll
-            for (int i = 0; i < GE::FPL; ++i) {
-                if (!on[i]) continue;
-                const Real ee = dexp(xv[u][i]);
-                if constexpr (MODEL == M_LOGISTIC) {
-                    const Real wt = ww[u][i];
-                    const Real pr = ee / (Real(1) + ee);
-                    acc[i].g += (double)(wt * pr * x);
-                    acc[i].h += (double)(wt * x * x * pr * (Real(1) - pr));
-                } else if constexpr (MODEL == M_POISSON) {
-                    const Real wt = ww[u][i];
-                    acc[i].g += (double)(wt * ee * x);
-                    acc[i].h += (double)(wt * ee * x * x);
-                } else {
-                    Real uu = ee * x;
-                    if constexpr (MODEL == M_COX) uu *= ww[u][i];
-                    const double uv = (double)uu;
-                    if (reset) {
-                        acc[i].t = 0.0;
-                        acc[i].r = 1.0;
+                for (int i = 0; i < GE::FPL; ++i) {
+                    if (dl[i] == 0.0) continue;
+                    const int f = sl + i * GE::SUB;
+                    const size_t q = (size_t)p * BP + f;
+                    const double dn = P.denom[q] + acc[i];
+                    P.denom[q] = dn;
+                    if constexpr (MODEL == M_CLR) {
+                        const double nw = P.nW[q];
+                        const double r = nw == 0.0 ? 0.0 : nw / dn;
+                        P.R1[q] = r;
+                        P.R2[q] = nw == 0.0 ? 0.0 : r / dn;
                     }
-                    const double sv = acc[i].t + uv;
-                    acc[i].l += r2[u][i] * uv * (2.0 * sv - uv);
-                    if (acc[i].r == 0.0) acc[i].m += r2[u][i] * uv;
-                    acc[i].t = sv;
-                    acc[i].g += uv * r1[u][i];
-                    acc[i].h += uv * (double)x * r1[u][i];
+                    acc[i] = 0.0;
                 }
             }
         }
@@ -330,22 +421,15 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t a
 }
 
 template <typename Real, int MODEL, int BP, int FMT>
-__device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t a, int64_t b,
-                                           const double* sDelta) {
+__device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t n0, int64_t n1,
+                                           const double (&dl)[Geo<BP>::FPL]) {
     using GE = Geo<BP>;
     constexpr int U = Unroll<BP>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
-    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
-    int64_t n0, n1;
-    subChunk<Real, FMT, GROUPED>(P, a, b, sg, GE::NSG, n0, n1);
+    const int sl = threadIdx.x % GE::SUB;
     double acc[GE::FPL];
-    double dl[GE::FPL];
 #pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) {
-        acc[i] = 0.0;
-        const int f = sl + i * GE::SUB;
-        dl[i] = f < P.B ? sDelta[f] : 0.0;
-    }
+    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
     for (int64_t e = n0; e < n1; e += U) {
         int64_t kk[U];
         Real xx[U], xv[U][GE::FPL], ww[U][GE::FPL];
@@ -423,6 +507,36 @@ __device__ __forceinline__ void combineSubgroups(Mon* sMon) {
         }
         __syncthreads();
     }
+}
+
+// Plain sums of (g, h) per fit over the sub-groups of a workgroup: a butterfly
+// over the lanes of a wave that hold the same fit, then the waves in order (a
+// fixed order, so results are reproducible). Result in sMon[f] (g, h).
+template <int BP>
+__device__ __forceinline__ void combinePlain(const Mon (&acc)[Geo<BP>::FPL], Mon* sMon) {
+    using GE = Geo<BP>;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int sl = threadIdx.x % GE::SUB;
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        double g = acc[i].g, h = acc[i].h;
+#pragma unroll
+        for (int off = GE::SUB; off < 64; off <<= 1) {
+            g += __shfl_xor(g, off, 64);
+            h += __shfl_xor(h, off, 64);
+        }
+        if (lane < GE::SUB) sMon[wave * BP + sl + i * GE::SUB] = Mon{g, h, 0, 0, 0, 0};
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < BP; f += BLOCK) {
+        Mon m = sMon[f];
+        for (int q = 1; q < BLOCK / 64; ++q) {
+            m.g += sMon[q * BP + f].g;
+            m.h += sMon[q * BP + f].h;
+        }
+        sMon[f] = m;
+    }
+    __syncthreads();
 }
 
 // Ordered combine of the G workgroup partials of fit f (slots) -> thread 0 gets it.
@@ -533,15 +647,52 @@ struct Shared {
 template <typename Real, int BP, class Fn>
 __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
                                            Shared<BP>& s, Fn fn, bool ordered = true) {
-    if (P.G == 1) {   // a one-workgroup team reduces in LDS only
-        if (threadIdx.x == 0)
-            for (int f = 0; f < P.B; ++f) fn(f, s.mon[f], s.out + 4 * f);
+    if (P.G == 1) {   // a one-workgroup team reduces in LDS only; fits step in parallel
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) fn(f, s.mon[f], s.out + 4 * f);
         __syncthreads();
         return true;
     }
     publish(P.epoch, ++seq, P.w);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     const double* slot = P.slots + (size_t)(seq & 1) * P.G * BP * SLOTW;
+    if (P.allRead && !ordered) {
+        // every workgroup sums every fit in one pass: thread t reads the partials
+        // w = t / BP, t / BP + BLOCK / BP, ... of fit t % BP; lanes of one fit are
+        // combined by a butterfly, then the waves (or parts) in order
+        constexpr int TPF = BLOCK / BP;
+        const int f = threadIdx.x % BP, part = threadIdx.x / BP, lane = threadIdx.x % 64;
+        double g = 0.0, h = 0.0;
+        if (f < P.B)
+            for (int w = part; w < P.G; w += TPF) {
+                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+                g += ald(q);
+                h += ald(q + 1);
+            }
+        int rowsOut;
+        if constexpr (BP < 64) {
+#pragma unroll
+            for (int off = BP; off < 64; off <<= 1) {
+                g += __shfl_xor(g, off, 64);
+                h += __shfl_xor(h, off, 64);
+            }
+            if (lane < BP) s.u.red[(threadIdx.x / 64) * BP + f] = Mon{g, h, 0, 0, 0, 0};
+            rowsOut = BLOCK / 64;
+        } else {
+            s.u.red[part * BP + f] = Mon{g, h, 0, 0, 0, 0};
+            rowsOut = TPF;
+        }
+        __syncthreads();
+        for (int ff = threadIdx.x; ff < P.B; ff += BLOCK) {
+            Mon m = s.u.red[ff];
+            for (int q = 1; q < rowsOut; ++q) {
+                m.g += s.u.red[q * BP + ff].g;
+                m.h += s.u.red[q * BP + ff].h;
+            }
+            fn(ff, m, s.out + 4 * ff);
+        }
+        __syncthreads();
+        return true;
+    }
     auto comb = [](const Mon& a, const Mon& b) { return monCombine(a, b); };
     // ordered combine of the G partials of fit f; plain sums (fixed shuffle tree)
     // when the partials carry no segment state
@@ -776,13 +927,10 @@ __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, 
             }
         }
     }
+    Mon m[GE::FPL];
 #pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) {
-        const int f = sl + i * GE::SUB;
-        s.mon[sg * BP + f] = Mon{acc[i], 0, 0, 0, 0, 0};
-    }
-    __syncthreads();
-    combineSubgroups<BP>(s.mon);
+    for (int i = 0; i < GE::FPL; ++i) m[i] = Mon{acc[i], 0, 0, 0, 0, 0};
+    combinePlain<BP>(m, s.mon);
 }
 
 // Per-pid initial state of this workgroup: nE, denominators, CLR risk ratios.
@@ -821,6 +969,34 @@ __device__ __forceinline__ void initPids(const Params<Real>& P) {
         }
     }
     __syncthreads();
+}
+
+// Per-coordinate metadata of column j for workgroup w: two table rows read with
+// vector loads (the vector-memory counter waits for them only where they are
+// used, one coordinate later; scalar loads would be waited for at the next LDS
+// access).
+struct Meta {
+    int j, fm, ptype, next;
+    int64_t nnz, a, b, c0;
+};
+__device__ __forceinline__ long long ldv(const long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename Real>
+__device__ __forceinline__ Meta loadMeta(const Params<Real>& P, int j, int w) {
+    const long long* ip = P.info + (size_t)j * 4;
+    const long long* rp = P.rng + ((size_t)j * P.G + w) * 2;
+    const long long w0 = ldv(ip), w1 = ldv(ip + 1);
+    Meta m;
+    m.j = j;
+    m.fm = (int)(w0 & 0xffffffffll);
+    m.ptype = ((w0 >> 32) & 1) ? 0 : P.priorType;
+    m.nnz = (int64_t)(unsigned)(w1 & 0xffffffffll);
+    m.next = (int)(w1 >> 32);
+    m.c0 = ldv(ip + 2);
+    m.a = ldv(rp);
+    m.b = ldv(rp + 1);
+    return m;
 }
 
 // The persistent findMode kernel (CyclicCoordinateDescent.cpp:909-1216 for every fit).
@@ -896,17 +1072,38 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
         for (int f = threadIdx.x; f < P.B; f += BLOCK) act |= s.active[f];
         allDone = !__syncthreads_or(act);
     }
+    // coordinates in cycle order: non-fixed columns; the metadata of the next one
+    // (and the beta/bound of the fits this workgroup steps) is fetched while the
+    // current one is in flight
+    const int firstJ = P.firstJ;
+    // fits whose step this workgroup computes: all (all-read) or the owned ones
+    auto mine = [&](int f) { return f < P.B && (own || f % P.G == w); };
+    Meta cur = loadMeta(P, firstJ < P.J ? firstJ : 0, w);
+    double preB = 0.0, preH = 0.0;   // thread f: beta / bound of fit f at cur.j
+    bool havePre = false;
+    // chunks of at most U entries stay in registers from the column pass to the
+    // update (narrow teams only: wide ones would hold too many registers)
+    constexpr bool FAST = BP <= 4;
+    constexpr int UF = 4;   // entries per sub-group kept in registers
+    auto fitsBatch = [&](const Meta& m) {   // every sub-group chunk of m has <= UF entries
+        return (m.b - m.a + GE::NSG - 1) / GE::NSG <= UF;
+    };
+    Batch<Real, MODEL, BP, UF> nb;   // prefetched row ids / values of cur (ungrouped models)
+    bool haveRows = false;
     for (int cycle = 0; !allDone && cycle < P.cycleBudget; ++cycle) {
-        for (int j = 0; j < P.J; ++j) {
-            if (P.fixed[j]) continue;
-            const int fm = P.fmt[j];
-            const int64_t nnzj = (fm == F_INDICATOR || fm == F_SPARSE) ? P.colPtr[j + 1] - P.colPtr[j] : P.K;
-            const int ptype = P.exclude[j] ? 0 : P.priorType;
-            if (own)
-                for (int f = threadIdx.x; f < P.B; f += BLOCK) {
-                    s.beta[f] = ald(&P.beta[(size_t)f * P.J + j]);
-                    s.hd[f] = ald(&P.hDelta[(size_t)f * P.J + j]);
+        while (firstJ < P.J) {
+            const int j = cur.j;
+            const int fm = cur.fm;
+            const int64_t nnzj = cur.nnz;
+            const int ptype = cur.ptype;
+            {
+                const int f = threadIdx.x;
+                if (mine(f)) {
+                    s.beta[f] = havePre ? preB : ald(&P.beta[(size_t)f * P.J + j]);
+                    s.hd[f] = havePre ? preH : ald(&P.hDelta[(size_t)f * P.J + j]);
                 }
+            }
+            const int jn = cur.next;
             // step of fit f from its reduced column pass: ccdUpdateBeta (:1394-1413),
             // prior (CovariatePrior.h), applyBounds (:1503-1520)
             auto step = [&](int f, const Mon& m, double* o) {
@@ -918,8 +1115,8 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     h = GROUPED ? m.h - m.l : m.h;
                 }
                 if (cDone[f]) { o[0] = 0.0; return; }
-                const double b = own ? s.beta[f] : ald(&P.beta[(size_t)f * P.J + j]);
-                double hd = own ? s.hd[f] : ald(&P.hDelta[(size_t)f * P.J + j]);
+                const double b = s.beta[f];
+                double hd = s.hd[f];
                 if (h < 0.0) { g = 0.0; h = 0.0; }
                 double delta = priorDeltaDev(ptype, P.var[f], g, h, b);
                 if (delta < -hd) delta = -hd;
@@ -931,91 +1128,159 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 }
                 o[0] = delta;
             };
-            if (nnzj == 0) {
-                // g = h = 0 without a pass (ModelSpecifics.hpp:720-723); identical everywhere
-                if (threadIdx.x == 0)
-                    for (int f = 0; f < P.B; ++f) {
-                        if (own) step(f, monZero(), s.out + 4 * f);
-                        else s.out[4 * f] = 0.0;
-                    }
-                __syncthreads();
-                if (!own) {
-                    // owner mode keeps beta/hDelta with the owners; an empty column still steps them
-                    if (threadIdx.x == 0)
-                        for (int f = w; f < P.B; f += P.G) {
-                            double o[4];
-                            step(f, monZero(), o);
-                            s.out[4 * f] = o[0];
+            __syncthreads();   // s.beta / s.hd written above
+            // prefetch for the next coordinate (a different column, so its beta and
+            // bound are final for this cycle)
+            const Meta nxt = loadMeta(P, jn, w);
+            havePre = jn != j;
+            if (havePre && mine(threadIdx.x)) {
+                preB = ald(&P.beta[(size_t)threadIdx.x * P.J + jn]);
+                preH = ald(&P.hDelta[(size_t)threadIdx.x * P.J + jn]);
+            }
+            // independent-row models: a small next column's row ids and values do not
+            // depend on this coordinate; they are fetched once this column's are used
+            const bool curRows = haveRows;
+            auto prefetchRows = [&]() {
+                if constexpr (FAST && !GROUPED) {
+                    haveRows = havePre && nxt.nnz != 0 && fitsBatch(nxt);
+                    if (haveRows) {
+                        const int sgq = threadIdx.x / GE::SUB;
+                        int64_t q0, q1;
+                        subChunk<Real, F_INTERCEPT, false>(P, nxt.a, nxt.b, sgq, GE::NSG, q0, q1);
+                        switch (nxt.fm) {
+                        case F_INDICATOR: loadRows<Real, MODEL, BP, F_INDICATOR, UF>(P, jn, q0, q1, nb); break;
+                        case F_SPARSE: loadRows<Real, MODEL, BP, F_SPARSE, UF>(P, jn, q0, q1, nb); break;
+                        case F_DENSE: loadRows<Real, MODEL, BP, F_DENSE, UF>(P, jn, q0, q1, nb); break;
+                        default: loadRows<Real, MODEL, BP, F_INTERCEPT, UF>(P, jn, q0, q1, nb); break;
                         }
-                    // deltas of other owners are not needed: an empty column changes no row
-                    __syncthreads();
+                    }
                 }
-                continue;
-            }
-            const unsigned long long tA = now();
-            int64_t a, b, c0;
-            colRange(P, j, w, a, b, c0);
-            Mon acc[GE::FPL];
-#pragma unroll
-            for (int i = 0; i < GE::FPL; ++i) acc[i] = monZero();
-            switch (fm) {
-            case F_INDICATOR: gradPass<Real, MODEL, BP, F_INDICATOR>(P, j, a, b, c0, acc, s.active); break;
-            case F_SPARSE: gradPass<Real, MODEL, BP, F_SPARSE>(P, j, a, b, c0, acc, s.active); break;
-            case F_DENSE: gradPass<Real, MODEL, BP, F_DENSE>(P, j, a, b, c0, acc, s.active); break;
-            default: gradPass<Real, MODEL, BP, F_INTERCEPT>(P, j, a, b, c0, acc, s.active); break;
-            }
-#pragma unroll
-            for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + sl + i * GE::SUB] = acc[i];
-            __syncthreads();
-            combineSubgroups<BP>(s.mon);
-            const unsigned long long tB = now();
-            writeSlots<Real, BP>(P, seq, s, GROUPED);
-            if (!reduceFits<Real, BP>(P, seq, deadline, s, step, GROUPED)) return;
-            const unsigned long long tC = now();
-            __shared__ double sDelta[BP];
-            int any = 0;
-            for (int f = threadIdx.x; f < BP; f += BLOCK) {
-                const double d = f < P.B ? s.out[4 * f] : 0.0;
-                sDelta[f] = d;
-                any |= (d != 0.0);
-            }
-            if (__syncthreads_or(any)) {
-                switch (fm) {
-                case F_INDICATOR: updatePass<Real, MODEL, BP, F_INDICATOR>(P, j, a, b, sDelta); break;
-                case F_SPARSE: updatePass<Real, MODEL, BP, F_SPARSE>(P, j, a, b, sDelta); break;
-                case F_DENSE: updatePass<Real, MODEL, BP, F_DENSE>(P, j, a, b, sDelta); break;
-                default: updatePass<Real, MODEL, BP, F_INTERCEPT>(P, j, a, b, sDelta); break;
-                }
+            };
+            if (nnzj == 0) {
+                // g = h = 0 without a pass (ModelSpecifics.hpp:720-723); every fit this
+                // workgroup steps moves its bound; no row changes
+                for (int f = threadIdx.x; f < P.B; f += BLOCK)
+                    if (mine(f)) step(f, monZero(), s.out + 4 * f);
+                prefetchRows();
                 __syncthreads();
-                if constexpr (MODEL == M_COX) {
-                    if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+            } else {
+                const unsigned long long tA = now();
+                const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+                int64_t n0, n1;
+                switch (fm) {
+                case F_INDICATOR: subChunk<Real, F_INDICATOR, GROUPED>(P, cur.a, cur.b, sg, GE::NSG, n0, n1); break;
+                case F_SPARSE: subChunk<Real, F_SPARSE, GROUPED>(P, cur.a, cur.b, sg, GE::NSG, n0, n1); break;
+                case F_DENSE: subChunk<Real, F_DENSE, GROUPED>(P, cur.a, cur.b, sg, GE::NSG, n0, n1); break;
+                default: subChunk<Real, F_INTERCEPT, GROUPED>(P, cur.a, cur.b, sg, GE::NSG, n0, n1); break;
+                }
+                bool on[GE::FPL];
+#pragma unroll
+                for (int i = 0; i < GE::FPL; ++i) {
+                    const int f = sl + i * GE::SUB;
+                    on[i] = f < P.B && s.active[f];
+                }
+                // a chunk of at most U entries stays in registers until the update
+                bool fast;
+                if constexpr (GROUPED) fast = FAST && __syncthreads_and(n1 - n0 <= UF);
+                else fast = FAST && fitsBatch(cur);
+                Batch<Real, MODEL, BP, UF> bt;
+                Mon acc[GE::FPL];
+#pragma unroll
+                for (int i = 0; i < GE::FPL; ++i) acc[i] = monZero();
+                if (FAST && fast) {
+                    if (!GROUPED && curRows) {
+                        bt = nb;   // row ids and values fetched during the previous coordinate
+                        loadState<Real, MODEL, BP, UF>(P, on, bt);
+                    } else {
+                        switch (fm) {
+                        case F_INDICATOR: loadBatch<Real, MODEL, BP, F_INDICATOR, UF>(P, j, n0, n1, on, bt); break;
+                        case F_SPARSE: loadBatch<Real, MODEL, BP, F_SPARSE, UF>(P, j, n0, n1, on, bt); break;
+                        case F_DENSE: loadBatch<Real, MODEL, BP, F_DENSE, UF>(P, j, n0, n1, on, bt); break;
+                        default: loadBatch<Real, MODEL, BP, F_INTERCEPT, UF>(P, j, n0, n1, on, bt); break;
+                        }
+                    }
+                    int prevSeg = 0;
+                    bool hasPrev = false;
+                    if constexpr (GROUPED) {
+                        if (n0 < n1 && n0 > cur.c0) {
+                            prevSeg = P.seg[fm == F_INDICATOR || fm == F_SPARSE ? P.rows[n0 - 1] : n0 - 1];
+                            hasPrev = true;
+                        }
+                    }
+                    accumBatch<Real, MODEL, BP, UF>(bt, on, acc, prevSeg, hasPrev);
+                } else {
+                    switch (fm) {
+                    case F_INDICATOR: gradPass<Real, MODEL, BP, F_INDICATOR>(P, j, n0, n1, cur.c0, acc, on); break;
+                    case F_SPARSE: gradPass<Real, MODEL, BP, F_SPARSE>(P, j, n0, n1, cur.c0, acc, on); break;
+                    case F_DENSE: gradPass<Real, MODEL, BP, F_DENSE>(P, j, n0, n1, cur.c0, acc, on); break;
+                    default: gradPass<Real, MODEL, BP, F_INTERCEPT>(P, j, n0, n1, cur.c0, acc, on); break;
+                    }
+                }
+                prefetchRows();
+                if constexpr (GROUPED) {
+#pragma unroll
+                    for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + sl + i * GE::SUB] = acc[i];
+                    __syncthreads();
+                    combineSubgroups<BP>(s.mon);
+                } else {
+                    combinePlain<BP>(acc, s.mon);
+                }
+                const unsigned long long tB = now();
+                writeSlots<Real, BP>(P, seq, s, GROUPED);
+                if (!reduceFits<Real, BP>(P, seq, deadline, s, step, GROUPED)) return;
+                const unsigned long long tC = now();
+                double dl[GE::FPL];
+                int any = 0;
+#pragma unroll
+                for (int i = 0; i < GE::FPL; ++i) {
+                    const int f = sl + i * GE::SUB;
+                    dl[i] = f < P.B ? s.out[4 * f] : 0.0;
+                    any |= (dl[i] != 0.0);
+                }
+                int nUpd = 0;
+                if (w == 0 && threadIdx.x == 0)
+                    for (int f = 0; f < P.B; ++f) nUpd += (s.out[4 * f] != 0.0) ? 1 : 0;
+                if (__syncthreads_or(any)) {
+                    if (FAST && fast) {
+                        applyBatch<Real, MODEL, BP, UF>(P, bt, dl);
+                    } else {
+                        switch (fm) {
+                        case F_INDICATOR: updatePass<Real, MODEL, BP, F_INDICATOR>(P, j, n0, n1, dl); break;
+                        case F_SPARSE: updatePass<Real, MODEL, BP, F_SPARSE>(P, j, n0, n1, dl); break;
+                        case F_DENSE: updatePass<Real, MODEL, BP, F_DENSE>(P, j, n0, n1, dl); break;
+                        default: updatePass<Real, MODEL, BP, F_INTERCEPT>(P, j, n0, n1, dl); break;
+                        }
+                    }
+                    __syncthreads();
+                    if constexpr (MODEL == M_COX) {
+                        if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+                    }
+                }
+                if (w == 0 && threadIdx.x == 0) {
+                    const unsigned long long tD = now();
+                    loc[1] += (double)(tB - tA);
+                    loc[2] += (double)(tC - tB);
+                    loc[3] += (double)(tD - tC);
+                    locSteps += 1;
+                    // compulsory bytes of this coordinate: the column pass reads ids /
+                    // values / weights once and xb of every active fit; the update reads
+                    // and writes xb of every fit that moved
+                    double nAct = 0;
+                    for (int f = 0; f < P.B; ++f) nAct += s.active[f] ? 1 : 0;
+                    const double n = (double)nnzj;
+                    const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
+                                             ((fm == F_SPARSE || fm == F_DENSE) ? sizeof(Real) : 0.0) +
+                                             ((P.rowFold || P.kw) ? 4.0 : 0.0));
+                    double by = base + n * nAct * sizeof(Real) + (GROUPED ? n * (8.0 + 16.0 * nAct) : 0.0);
+                    if (nUpd > 0) {
+                        by += base + n * nUpd * 2.0 * sizeof(Real) + (GROUPED ? n * (4.0 + 16.0 * nUpd) : 0.0);
+                        if (MODEL == M_COX) by += (double)P.N * P.B * 56.0;
+                    }
+                    loc[0] += by;
                 }
             }
-            if (w == 0 && threadIdx.x == 0) {
-                const unsigned long long tD = now();
-                loc[1] += (double)(tB - tA);
-                loc[2] += (double)(tC - tB);
-                loc[3] += (double)(tD - tC);
-                locSteps += 1;
-                // compulsory bytes of this coordinate (my algorithm): the column pass
-                // reads ids/values/weights once and xb of every active fit; the update
-                // reads and writes xb of every fit that moved
-                double nAct = 0, nUpd = 0;
-                for (int f = 0; f < P.B; ++f) {
-                    nAct += s.active[f] ? 1 : 0;
-                    nUpd += (sDelta[f] != 0.0) ? 1 : 0;
-                }
-                const double n = (double)nnzj;
-                const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
-                                         ((fm == F_SPARSE || fm == F_DENSE) ? sizeof(Real) : 0.0) +
-                                         ((P.rowFold || P.kw) ? 4.0 : 0.0));
-                double b = base + n * nAct * sizeof(Real) + (GROUPED ? n * (8.0 + 16.0 * nAct) : 0.0);
-                if (nUpd > 0) {
-                    b += base + n * nUpd * 2.0 * sizeof(Real) + (GROUPED ? n * (4.0 + 16.0 * nUpd) : 0.0);
-                    if (MODEL == M_COX) b += (double)P.N * P.B * 56.0;
-                }
-                loc[0] += b;
-            }
+            cur = nxt;
+            if (jn <= j) break;   // wrapped: the cycle is complete
         }
         // convergence test of every fit (performCheckConvergence :839-906)
         const unsigned long long tO = now();
