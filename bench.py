@@ -7,7 +7,8 @@ One step = one batched launch that fits B models of that data from a cold start:
 4 row folds (BY_ROW selection, each fit drops one fold) x B/4 prior variances on a
 log grid around 1.0, FP32 bulk arithmetic, convergence on relative log-likelihood
 change < 1e-6. The plain per-fit path (host CCD loop, one launch per engine call)
-and the single-fit resident kernel are timed on one member of the same mix.
+and the single-fit resident kernel are timed on one member of the same mix over a
+fixed number of cycles and extrapolated to the cycles that member needs.
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--batch 64]
 """
@@ -108,39 +109,45 @@ def main():
     w_ref = (rf != held[ref_i]).astype(float)
     prior_ref = create_prior("normal", var_list[ref_i])
 
-    # ---- resident single fit (converged) ----------------------------------------
-    t0 = time.perf_counter()
-    b_res, r_res = eng.fit(prior_ref, ctl, weights=w_ref, mode="resident")
-    t_resident = time.perf_counter() - t0
-    log(f"resident single fit: {t_resident:.2f}s, {r_res['iterations']} iterations")
-    scale_ref = max(np.max(np.abs(b_res)), 1e-12)
-    batched_vs_resident = float(np.max(np.abs(betas[ref_i] - b_res)) / scale_ref)
-
-    # ---- plain per-fit path ------------------------------------------------------
-    # Every cycle of the plain path performs the same launches (one gradient and one
-    # update per coordinate), so it is timed over `plain_cycles` cycles and
-    # extrapolated to the cycle count of the converged fit; the resident kernel run
-    # for the same cycles must give identical coefficients.
-    plain = None
+    it_ref = res[ref_i]["iterations"]
+    # ---- single fits: plain per-fit path and one-fit resident kernel -------------
+    # Every cycle of a single fit performs the same work (one column pass and one
+    # update per coordinate), so both are timed over `plain_cycles` cycles of the
+    # reference member of the mix and extrapolated to the cycles that member needed
+    # in the batch; run for the same cycles, both must give identical coefficients,
+    # and the batched member must equal them when the batch is capped likewise.
+    plain = resident = None
     if not args.skip_plain:
         cyc = args.plain_cycles
         ctl_cap = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=cyc)
         t0 = time.perf_counter()
         b_plain, r_plain = eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="plain")
         t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
         b_cap, r_cap = eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="resident")
+        t_res = time.perf_counter() - t0
+        b_bcap, _ = eng.fit_batch(var_list, "normal", ctl_cap, held_out_fold=held, row_fold=rf)
         per_cycle = t_plain / r_plain["iterations"]
-        t_fit = per_cycle * r_res["iterations"]
+        t_fit = per_cycle * it_ref
+        scale = max(np.max(np.abs(b_cap)), 1e-12)
         plain = {
             "seconds_per_fit": t_fit,
             "fits_per_sec": 1.0 / t_fit,
             "seconds_per_cycle": per_cycle,
             "timed_cycles": r_plain["iterations"],
-            "extrapolated_to_cycles": r_res["iterations"],
-            "plain_vs_resident_same_cycles_max_coef_rel_diff":
-                float(np.max(np.abs(b_plain - b_cap)) / max(np.max(np.abs(b_cap)), 1e-12)),
+            "extrapolated_to_cycles": it_ref,
+            "plain_vs_resident_same_cycles_max_coef_rel_diff": float(np.max(np.abs(b_plain - b_cap)) / scale),
         }
-        log(f"plain per-fit path: {per_cycle:.3f}s per cycle over {r_plain['iterations']} cycles")
+        rpc = t_res / r_cap["iterations"]
+        resident = {
+            "seconds_per_fit": rpc * it_ref,
+            "fits_per_sec": 1.0 / (rpc * it_ref),
+            "seconds_per_cycle": rpc,
+            "timed_cycles": r_cap["iterations"],
+            "batch_member_vs_resident_same_cycles_max_coef_rel_diff": float(np.max(np.abs(b_bcap[ref_i] - b_cap)) / scale),
+        }
+        log(f"plain per-fit path: {per_cycle:.3f}s per cycle; resident single fit {rpc:.3f}s per cycle "
+            f"(over {r_plain['iterations']} cycles)")
     eng.close()
 
     # ---- CPU baseline: the oracle (FP64, one core) on the same data --------------
@@ -150,10 +157,9 @@ def main():
         cyc = args.cpu_cycles
         sec, _ = oracle.time_cycles(d, prior_ref, ctl, cyc)
         per_cycle = sec / cyc
-        it = r_res["iterations"]
-        cpu = {"value": 1.0 / (per_cycle * it), "unit": "fits/s", "cores": 1, "kind": "port",
+        cpu = {"value": 1.0 / (per_cycle * it_ref), "unit": "fits/s", "cores": 1, "kind": "port",
                "sample": f"{cyc} CCD cycle(s) of one fit of this workload in FP64 on one host core "
-                         f"({per_cycle:.2f}s per cycle), extrapolated to the {it} cycles the fit needs"}
+                         f"({per_cycle:.2f}s per cycle), extrapolated to the {it_ref} cycles the fit needs"}
         log(f"cpu baseline: {per_cycle:.2f}s per cycle")
 
     lat = np.array(step_times) * 1e3
@@ -176,9 +182,7 @@ def main():
                    "return_flags": sorted(set(flags))},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "note": "batched fits complete together; per-fit latency = step time"},
-        "resident_single_fit": {"seconds": t_resident, "fits_per_sec": 1.0 / t_resident,
-                                "iterations": r_res["iterations"],
-                                "batched_vs_resident_max_coef_rel_diff": batched_vs_resident},
+        "resident_single_fit": resident,
         "plain_per_fit": plain,
         "speedup_vs_plain": (fits_per_sec / plain["fits_per_sec"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -4,7 +4,9 @@
 
 #include "resident_kernels.h"
 
+#include <array>
 #include <chrono>
+#include <map>
 #include <numeric>
 #include <cstdlib>
 #include <cstdio>
@@ -153,6 +155,7 @@ struct ResWork {
     double* stats = nullptr;
     Real* kw = nullptr;
     res::Params<Real>* teams = nullptr;
+    std::map<int, std::array<int, 3>> tuned;   // batch size -> {Bt, T, nTeams} picked by timing
 
     void freeBatch() {
         dfree(xb); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
@@ -218,9 +221,12 @@ static Layout chooseLayout(int64_t K, int B, int device) {
     } else if (bt > 0 && t > 0) {
         L.Bt = std::min(bt, 256);
         L.T = std::min(t, res::MAXG);
-    } else {        // independent fits: one workgroup each, no grid exchange at all
+    } else if (K <= 200000) {   // short data: independent one-workgroup fits, no exchange
         L.Bt = 1;
         L.T = 1;
+    } else {        // teams of 4 fits on about 32k rows per workgroup
+        L.Bt = 4;
+        L.T = (int)std::min<int64_t>(64, std::max<int64_t>(8, K / 32768));
     }
     L.nTeams = (B + L.Bt - 1) / L.Bt;
     return L;
@@ -244,6 +250,47 @@ static int chunkFits(const Layout& L, int64_t K, int64_t N, bool grouped, int re
 // per model in resident_<model>.hip so the instantiations compile in parallel.
 template <typename Real, int MODEL>
 void launchByBP(int BP, const res::Params<Real>* teams, int nTeams, int T, hipStream_t s);
+template <typename Real, int MODEL>
+int occupancyByBP(int BP);
+
+template <typename Real>
+static int occupancyOf(int model, int BP) {
+    switch (model) {
+    case M_LOGISTIC: return occupancyByBP<Real, M_LOGISTIC>(BP);
+    case M_POISSON: return occupancyByBP<Real, M_POISSON>(BP);
+    case M_CLR: return occupancyByBP<Real, M_CLR>(BP);
+    default: return occupancyByBP<Real, M_COX>(BP);
+    }
+}
+
+// Team shapes worth timing for a batch of B fits on K rows: one lockstep team of
+// every fit, and narrower teams (4, 16 or 64 fits) of 8 to 128 workgroups, each
+// shape co-resident on the device and at least 2048 rows per workgroup; plus
+// one-workgroup teams for short data.
+template <typename Real>
+static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int device) {
+    int cus = 0;
+    CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    std::vector<Layout> out;
+    auto add = [&](int bt, int t) {
+        Layout L;
+        L.Bt = bt;
+        L.T = t;
+        L.nTeams = (B + bt - 1) / bt;
+        const int cap = occupancyOf<Real>(model, padFits(bt)) * cus;
+        if (t > 1 && (int64_t)L.nTeams * t > cap) return;
+        for (const Layout& q : out) if (q.Bt == L.Bt && q.T == L.T) return;
+        out.push_back(L);
+    };
+    if (B <= 256) add(B, chooseWorkgroups(K, B, device));
+    for (int bt : {4, 16, 64}) {
+        if (bt >= B) continue;
+        for (int t = 8; t <= 128; t *= 2)
+            if (K / t >= 2048) add(bt, t);
+    }
+    if (K <= 200000) { add(1, 1); add(4, 1); }
+    return out;
+}
 
 // Run fits [0, B) (host arrays indexed by fit) as persistent launches until all
 // have finished; fills betas/flags/iters/ll. kwHost: row weights of a single fit.
@@ -633,13 +680,56 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     std::iota(pending.begin(), pending.end(), 0);
     const bool grouped = D->model == M_CLR || D->model == M_COX;
     const int64_t nGroups = D->model == M_CLR ? D->nStrata : D->K;
+    // Team shape of the whole batch: CCD_BATCH_LAYOUT, else the fastest candidate
+    // on one timed cycle of these fits (kept per batch size), else the default.
+    bool tunedOk = false;
+    Layout LT;
+    if (n > 2 && !std::getenv("CCD_BATCH_LAYOUT") && !(std::getenv("CCD_AUTOTUNE") && std::atoi(std::getenv("CCD_AUTOTUNE")) == 0)) {
+        if (!res) res = new ResWork<Real>();
+        auto& W = *static_cast<ResWork<Real>*>(res);
+        auto it = W.tuned.find(n);
+        if (it != W.tuned.end()) {
+            LT.Bt = it->second[0];
+            LT.T = it->second[1];
+            LT.nTeams = it->second[2];
+            tunedOk = true;
+        } else {
+            const std::vector<Layout> cand = layoutCandidates<Real>(D->model, D->K, n, D->device);
+            if (cand.size() > 1) {
+                ccd_fit_args one = a;
+                one.max_iterations = 1;
+                std::vector<double> v(n), bd(n, a.initial_bound), l(n);
+                std::vector<int32_t> h(n);
+                std::vector<int> fl(n), it2(n);
+                for (int f = 0; f < n; ++f) {
+                    v[f] = bargs.variances[f];
+                    h[f] = bargs.held_out_fold ? bargs.held_out_fold[f] : -1;
+                }
+                double best = 1e300;
+                for (const Layout& L : cand) {
+                    std::vector<double> bb(betas, betas + (size_t)n * J);
+                    double ms = 0, bytes = 0, steps = 0;
+                    int nl = 0;
+                    runResident<Real>(*this, one, n, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(),
+                                      bb.data(), fl.data(), it2.data(), l.data(), ms, bytes, steps, nl, L);
+                    if (std::getenv("CCD_PROGRESS"))
+                        std::fprintf(stderr, "[ccd] autotune %d fits: %d fits x %d workgroups per team: %.1f ms per cycle\n",
+                                     n, L.Bt, L.T, ms);
+                    if (ms < best) { best = ms; LT = L; }
+                }
+                W.tuned[n] = {LT.Bt, LT.T, LT.nTeams};
+                tunedOk = true;
+            }
+        }
+    }
     while (!pending.empty()) {
         std::vector<int> retry;
-        const Layout L0 = chooseLayout(D->K, (int)pending.size(), D->device);
-        const size_t chunk = (size_t)chunkFits(L0, D->K, nGroups, grouped, (int)sizeof(Real));
+        const bool whole = tunedOk && (int)pending.size() == n;
+        const Layout L0 = whole ? LT : chooseLayout(D->K, (int)pending.size(), D->device);
+        const size_t chunk = whole ? (size_t)n : (size_t)chunkFits(L0, D->K, nGroups, grouped, (int)sizeof(Real));
         for (size_t c0 = 0; c0 < pending.size(); c0 += chunk) {
             const int B = (int)std::min(chunk, pending.size() - c0);
-            const Layout L = chooseLayout(D->K, B, D->device);
+            const Layout L = whole ? LT : chooseLayout(D->K, B, D->device);
             std::vector<double> v(B), bd(B), bb((size_t)B * J), l(B);
             std::vector<int32_t> h(B);
             std::vector<int> fl(B), it(B);

@@ -36,10 +36,27 @@ void launchByBP(int BP, const res::Params<Real>* teams, int nTeams, int T, hipSt
     }
 }
 
+// Workgroups of kFindMode<Real, MODEL, BP> that one compute unit holds at once.
+template <typename Real, int MODEL>
+int occupancyByBP(int BP) {
+    int perCU = 0;
+    switch (BP) {
+    case 1: CCD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, res::kFindMode<Real, MODEL, 1>, BLOCK, 0)); break;
+    case 4: CCD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, res::kFindMode<Real, MODEL, 4>, BLOCK, 0)); break;
+    case 16: CCD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, res::kFindMode<Real, MODEL, 16>, BLOCK, 0)); break;
+    case 64: CCD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, res::kFindMode<Real, MODEL, 64>, BLOCK, 0)); break;
+    case 256: CCD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, res::kFindMode<Real, MODEL, 256>, BLOCK, 0)); break;
+    default: throw std::invalid_argument("unsupported fit padding");
+    }
+    return perCU;
+}
+
 }  // namespace ccd
 
 #define CCD_INSTANTIATE_LAUNCH(MODEL)                                                                   \
     namespace ccd {                                                                                     \
     template void launchByBP<float, MODEL>(int, const res::Params<float>*, int, int, hipStream_t);     \
     template void launchByBP<double, MODEL>(int, const res::Params<double>*, int, int, hipStream_t);   \
+    template int occupancyByBP<float, MODEL>(int);                                                      \
+    template int occupancyByBP<double, MODEL>(int);                                                     \
     }
