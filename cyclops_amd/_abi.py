@@ -6,6 +6,7 @@ src/cyclops/CompressedDataMatrix.h:36-38) so values can be passed through unchan
 import ctypes as C
 
 CCD_OK = 0
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED = 1, 2, 3, 4, 5
 
 # ModelType (src/cyclops/Types.h:134-147)
 POISSON = 2

@@ -50,6 +50,8 @@ ccd::EngineBase& E(ccd_handle h) {
 
 namespace ccd {
 
+void setLastError(const std::string& msg) { g_err = msg; }
+
 // ---------------------------------------------------------------------------
 // Priors (CovariatePrior.h:186-188 NoPrior, 240-273 LaplacePrior, 410-415 NormalPrior)
 // ---------------------------------------------------------------------------

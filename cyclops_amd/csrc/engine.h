@@ -93,6 +93,16 @@ public:
 
 EngineBase* createEngine(const ccd_data_desc& d, int device);
 
+// What a one-fit persistent launch over `d` needs on `device`: its workgroups, the
+// fraction of the device's co-resident capacity they take, and the HBM of its
+// workspace beyond the engine itself (admission control of the server).
+struct ResidentNeeds {
+    int workgroups;
+    double capacityFraction;
+    double workspaceBytes;
+};
+ResidentNeeds residentNeeds(const ccd_data_desc& d, int device);
+
 // Host CCD driver over an EngineBase: CyclicCoordinateDescent::update/findMode
 // (CyclicCoordinateDescent.cpp:586-615, 909-1216). This is the plain per-fit path.
 void fitPlain(EngineBase& eng, const ccd_fit_args& a, double* beta, ccd_fit_result& r);
@@ -103,5 +113,8 @@ void assignFolds(int64_t K, const int32_t* pid, int fold, int selector, int64_t 
 // Prior step (CovariatePrior.h:186-188, 240-273, 410-415), shared by host drivers.
 double priorDelta(int type, double variance, double g, double h, double beta);
 double priorLogDensity(int type, double variance, double beta);
+
+// Message returned by ccd_last_error() on the calling thread.
+void setLastError(const std::string& msg);
 
 }  // namespace ccd

@@ -658,6 +658,21 @@ public:
 
 namespace ccd {
 
+ResidentNeeds residentNeeds(const ccd_data_desc& d, int device) {
+    ResidentNeeds n{};
+    const int G = chooseWorkgroups(d.K, 1, device);
+    int cus = 0;
+    CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int occ = d.precision == CCD_FP32 ? occupancyOf<float>(d.model_type, 1) : occupancyOf<double>(d.model_type, 1);
+    n.workgroups = G;
+    n.capacityFraction = (double)G / std::max(1, occ * cus);
+    const double s = d.precision == CCD_FP32 ? 4.0 : 8.0;
+    const bool grouped = d.model_type == CCD_MODEL_CONDITIONAL_LOGISTIC || d.model_type == CCD_MODEL_COX;
+    n.workspaceBytes = d.K * (s + 12.0) + (grouped ? (d.K + 1) * 5.0 * 8.0 : 0.0) +
+                       (double)d.J * ((G + 1) * 4.0 + G * 16.0 + 32.0 * 8.0 + 64.0) + (1 << 20);
+    return n;
+}
+
 EngineBase* createEngine(const ccd_data_desc& d, int device) {
     if (d.precision == CCD_FP64) return new EngineT<double>(makeDevData<double>(d, device));
     if (d.precision == CCD_FP32) return new EngineT<float>(makeDevData<float>(d, device));

@@ -1,13 +1,293 @@
-// server.hip -- request loop (filled in below).
+// server.hip -- the fit-serving loop behind ccd_server_* (include/cyclops_amd.h).
+//
+// Continuous admission: `slots` worker threads each own a HIP stream (one engine
+// per request). A request is admitted, first-fit in arrival order, as soon as a
+// slot is free, its HBM footprint fits what the running requests leave of the
+// budget, and (resident mode) its persistent launch fits what they leave of the
+// device's co-resident workgroups -- so concurrent persistent kernels can always
+// all be resident and never wait on each other. Admitted fits run concurrently on
+// the device. A request's kernel shape depends only on its own data, so its result
+// is the same alone and in any mix.
+//
+// The reference has no serving loop; its concurrency is thread-level clones of
+// the engine (AbstractCrossValidationDriver.cpp:44-129, Thread.h:75-130), which
+// this generalises to independent requests.
 #include "engine.h"
 
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+enum ReqState { QUEUED = 0, RUNNING = 1, DONE = 2 };
+
+struct Request {
+    int64_t id = 0;
+    // owned copies of the caller's description and arguments
+    std::vector<int32_t> format, rows, pid;
+    std::vector<int64_t> colPtr, densePtr;
+    std::vector<double> values, dense, y, time, weights, beta;
+    std::vector<uint8_t> exclude, fixed;
+    ccd_data_desc desc{};
+    ccd_fit_args args{};
+    int32_t J = 0;
+    // admission
+    double footprint = 0.0;
+    double capFraction = 0.0;
+    // outcome
+    ReqState state = QUEUED;
+    int rc = CCD_OK;
+    std::string error;
+    ccd_fit_result result{};
+    ccd_request_stats stats{};
+};
+
+template <class T>
+void copyIn(std::vector<T>& dst, const T* src, size_t n) {
+    if (src) dst.assign(src, src + n);
+    else dst.clear();
+}
+
+}  // namespace
+
+struct ccd_server {
+    ccd_server_config cfg{};
+    std::chrono::steady_clock::time_point t0;
+    std::mutex m;
+    std::condition_variable work;   // a request may be admissible / stop
+    std::condition_variable done;   // a request finished
+    std::deque<std::shared_ptr<Request>> queue;
+    std::map<int64_t, std::shared_ptr<Request>> all;
+    double usedBytes = 0.0, usedCap = 0.0;
+    int64_t nextId = 1;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // First queued request that fits the remaining budget (caller holds m).
+    std::shared_ptr<Request> pick() {
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            const auto& r = *it;
+            if (usedBytes + r->footprint > cfg.hbm_budget_bytes) continue;
+            if (usedCap + r->capFraction > 1.0 + 1e-9) continue;
+            auto q = r;
+            queue.erase(it);
+            return q;
+        }
+        return nullptr;
+    }
+
+    void run(Request& r) {
+        CCD_HIP(hipSetDevice(cfg.device));
+        ccd_handle h = nullptr;
+        int rc = ccd_create(&r.desc, cfg.device, &h);
+        if (rc == CCD_OK) {
+            // the engine holds its own copy in HBM: release the host copy now
+            std::vector<int32_t>().swap(r.rows);
+            std::vector<double>().swap(r.values);
+            std::vector<double>().swap(r.dense);
+            rc = ccd_fit(h, &r.args, cfg.mode, r.beta.data(), &r.result);
+        }
+        if (rc != CCD_OK) r.error = ccd_last_error();
+        if (h) ccd_destroy(h);
+        r.rc = rc;
+    }
+
+    void worker() {
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                work.wait(lk, [&] { return (r = pick()) != nullptr || (stopping && queue.empty()); });
+                if (!r) return;
+                usedBytes += r->footprint;
+                usedCap += r->capFraction;
+                r->state = RUNNING;
+                r->stats.start_time = now();
+            }
+            try {
+                run(*r);
+            } catch (const std::exception& ex) {
+                r->rc = CCD_ERR_HIP;
+                r->error = ex.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(m);
+                usedBytes -= r->footprint;
+                usedCap -= r->capFraction;
+                r->stats.end_time = now();
+                r->state = DONE;
+                // keep only what poll returns
+                std::vector<int32_t>().swap(r->format);
+                std::vector<int64_t>().swap(r->colPtr);
+                std::vector<int64_t>().swap(r->densePtr);
+                std::vector<double>().swap(r->y);
+                std::vector<double>().swap(r->time);
+                std::vector<int32_t>().swap(r->pid);
+                std::vector<double>().swap(r->weights);
+            }
+            done.notify_all();
+            work.notify_all();
+        }
+    }
+};
+
+namespace {
+// Errors go to the same per-thread message as every other entry point (ccd_last_error).
+template <class F>
+int sguard(F&& f) {
+    try {
+        return f();
+    } catch (const ccd::HipError& ex) {
+        ccd::setLastError(ex.what());
+        return CCD_ERR_HIP;
+    } catch (const std::invalid_argument& ex) {
+        ccd::setLastError(ex.what());
+        return CCD_ERR_INVALID;
+    } catch (const std::bad_alloc&) {
+        ccd::setLastError("out of host memory");
+        return CCD_ERR_NOMEM;
+    } catch (const std::exception& ex) {
+        ccd::setLastError(ex.what());
+        return CCD_ERR_STATE;
+    }
+}
+}  // namespace
+
 extern "C" {
-int ccd_server_start(const ccd_server_config*, ccd_server_handle*) { return CCD_ERR_UNSUPPORTED; }
-int ccd_server_submit(ccd_server_handle, const ccd_data_desc*, const ccd_fit_args*, const double*, int64_t*) {
-    return CCD_ERR_UNSUPPORTED;
+
+int ccd_server_start(const ccd_server_config* cfg, ccd_server_handle* out) {
+    return sguard([&] {
+        if (!cfg || !out) throw std::invalid_argument("null argument");
+        int n = 0;
+        CCD_HIP(hipGetDeviceCount(&n));
+        if (cfg->device < 0 || cfg->device >= n) throw std::invalid_argument("no such device");
+        if (cfg->slots < 1 || cfg->slots > 64) throw std::invalid_argument("slots must be in [1, 64]");
+        if (!(cfg->hbm_budget_bytes > 0)) throw std::invalid_argument("hbm_budget_bytes must be > 0");
+        if (cfg->mode != CCD_MODE_PLAIN && cfg->mode != CCD_MODE_RESIDENT) throw std::invalid_argument("bad mode");
+        auto* s = new ccd_server();
+        s->cfg = *cfg;
+        s->t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < cfg->slots; ++i) s->workers.emplace_back([s] { s->worker(); });
+        *out = s;
+        return CCD_OK;
+    });
 }
-int ccd_server_poll(ccd_server_handle, int64_t, int, double*, ccd_fit_result*, ccd_request_stats*) {
-    return CCD_ERR_UNSUPPORTED;
+
+int ccd_server_submit(ccd_server_handle s, const ccd_data_desc* d, const ccd_fit_args* a, const double* beta0,
+                      int64_t* request_id) {
+    return sguard([&] {
+        if (!s || !d || !a || !request_id) throw std::invalid_argument("null argument");
+        if (d->K <= 0 || d->J <= 0 || !d->format || !d->col_ptr || !d->y)
+            throw std::invalid_argument("incomplete data description");
+        auto r = std::make_shared<Request>();
+        const int64_t K = d->K;
+        const int32_t J = d->J;
+        r->J = J;
+        copyIn(r->format, d->format, J);
+        copyIn(r->colPtr, d->col_ptr, J + 1);
+        const int64_t nnz = d->col_ptr[J];
+        if (nnz < 0) throw std::invalid_argument("col_ptr[J] < 0");
+        copyIn(r->rows, d->rows, nnz);
+        copyIn(r->values, d->values, nnz);
+        if (d->dense_ptr) {
+            copyIn(r->densePtr, d->dense_ptr, J + 1);
+            copyIn(r->dense, d->dense_values, d->dense_ptr[J]);
+        }
+        copyIn(r->y, d->y, K);
+        copyIn(r->time, d->time, K);
+        copyIn(r->pid, d->pid, K);
+        r->desc = *d;
+        r->desc.format = r->format.data();
+        r->desc.col_ptr = r->colPtr.data();
+        r->desc.rows = r->rows.empty() ? nullptr : r->rows.data();
+        r->desc.values = r->values.empty() ? nullptr : r->values.data();
+        r->desc.dense_ptr = r->densePtr.empty() ? nullptr : r->densePtr.data();
+        r->desc.dense_values = r->dense.empty() ? nullptr : r->dense.data();
+        r->desc.y = r->y.data();
+        r->desc.time = r->time.empty() ? nullptr : r->time.data();
+        r->desc.pid = r->pid.empty() ? nullptr : r->pid.data();
+        r->args = *a;
+        copyIn(r->exclude, a->exclude, J);
+        copyIn(r->fixed, a->fixed, J);
+        copyIn(r->weights, a->weights, K);
+        r->args.exclude = r->exclude.empty() ? nullptr : r->exclude.data();
+        r->args.fixed = r->fixed.empty() ? nullptr : r->fixed.data();
+        r->args.weights = r->weights.empty() ? nullptr : r->weights.data();
+        r->beta.assign(J, 0.0);
+        if (beta0) std::copy(beta0, beta0 + J, r->beta.begin());
+        // admission footprint: the engine plus the resident workspace
+        double bytes = 0.0;
+        if (ccd_estimate_footprint(&r->desc, &bytes) != CCD_OK) throw std::invalid_argument(ccd_last_error());
+        if (s->cfg.mode == CCD_MODE_RESIDENT) {
+            const ccd::ResidentNeeds n = ccd::residentNeeds(r->desc, s->cfg.device);
+            bytes += n.workspaceBytes;
+            r->capFraction = n.capacityFraction;
+        }
+        r->footprint = bytes;
+        if (bytes > s->cfg.hbm_budget_bytes)
+            throw std::invalid_argument("request needs " + std::to_string(bytes) + " bytes of HBM, over the budget");
+        {
+            std::lock_guard<std::mutex> lk(s->m);
+            if (s->stopping) throw std::runtime_error("server is stopping");
+            r->id = s->nextId++;
+            r->stats.submit_time = s->now();
+            r->stats.footprint_bytes = bytes;
+            s->queue.push_back(r);
+            s->all[r->id] = r;
+            *request_id = r->id;
+        }
+        s->work.notify_all();
+        return CCD_OK;
+    });
 }
-int ccd_server_stop(ccd_server_handle) { return CCD_ERR_UNSUPPORTED; }
+
+int ccd_server_poll(ccd_server_handle s, int64_t id, int wait, double* beta_out, ccd_fit_result* result,
+                    ccd_request_stats* stats) {
+    return sguard([&] {
+        if (!s) throw std::invalid_argument("null server handle");
+        std::unique_lock<std::mutex> lk(s->m);
+        auto it = s->all.find(id);
+        if (it == s->all.end()) throw std::invalid_argument("unknown request id");
+        auto r = it->second;
+        if (r->state != DONE) {
+            if (!wait) {
+                ccd::setLastError("request still running");
+                return CCD_ERR_STATE;
+            }
+            s->done.wait(lk, [&] { return r->state == DONE; });
+        }
+        if (stats) *stats = r->stats;
+        if (result) *result = r->result;
+        if (beta_out) std::copy(r->beta.begin(), r->beta.end(), beta_out);
+        if (r->rc != CCD_OK) ccd::setLastError(r->error);
+        return r->rc;
+    });
 }
+
+int ccd_server_stop(ccd_server_handle s) {
+    return sguard([&] {
+        if (!s) throw std::invalid_argument("null server handle");
+        {
+            std::lock_guard<std::mutex> lk(s->m);
+            s->stopping = true;   // queued requests still run; new ones are refused
+        }
+        s->work.notify_all();
+        for (auto& t : s->workers) t.join();
+        delete s;
+        return CCD_OK;
+    });
+}
+
+}  // extern "C"

@@ -280,3 +280,58 @@ def assign_folds(K, pid, fold, selector, seed):
     check(lib().ccd_cv_assign_folds(K, _abi.ptr(p, C.c_int32), fold, selector, seed,
                                     out.ctypes.data_as(C.POINTER(C.c_int32))))
     return out
+
+
+class Server:
+    """Fit-serving loop (ccd_server_*): requests of any model and size are admitted
+    by HBM budget and run concurrently on the device; poll() returns each result
+    with its submit / start / end times."""
+
+    def __init__(self, slots=4, hbm_budget_bytes=200e9, mode="resident", device=0):
+        cfg = _abi.ServerConfig()
+        cfg.device = device
+        cfg.slots = slots
+        cfg.hbm_budget_bytes = float(hbm_budget_bytes)
+        cfg.mode = {"plain": _abi.MODE_PLAIN, "resident": _abi.MODE_RESIDENT}[mode]
+        h = C.c_void_p()
+        check(lib().ccd_server_start(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._J = {}
+
+    def submit(self, data, prior=None, control=None, precision=_abi.FP64, weights=None, fixed=None, beta0=None):
+        prior = prior or create_prior("laplace", 1.0)
+        control = control or create_control()
+        desc = data.desc(precision)
+        hold = FitArgsHolder(data.J, prior, control, weights=weights, fixed=fixed,
+                             intercept_index=intercept_index(data))
+        b0 = None if beta0 is None else np.ascontiguousarray(beta0, dtype=np.float64)
+        rid = C.c_int64()
+        check(lib().ccd_server_submit(self.h, C.byref(desc), C.byref(hold.args),
+                                      None if b0 is None else _d(b0), C.byref(rid)))
+        self._J[rid.value] = data.J
+        return rid.value
+
+    def poll(self, request_id, wait=True):
+        """(beta, result, stats), or None while the request runs and wait is False."""
+        J = self._J.get(request_id, 0)
+        beta = np.zeros(J)
+        res = _abi.FitResult()
+        st = _abi.RequestStats()
+        rc = lib().ccd_server_poll(self.h, request_id, int(wait), _d(beta) if J else None, C.byref(res),
+                                   C.byref(st))
+        if rc == _abi.ERR_STATE and not wait:
+            return None
+        check(rc)
+        return beta, res.as_dict(), {"submit": st.submit_time, "start": st.start_time, "end": st.end_time,
+                                     "footprint_bytes": st.footprint_bytes}
+
+    def stop(self):
+        if getattr(self, "h", None):
+            check(lib().ccd_server_stop(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

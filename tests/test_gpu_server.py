@@ -1,0 +1,97 @@
+"""GPU tests of the serving loop (ccd_server_*): every result is the same alone and
+in a concurrent mix of model types and sizes, admission honours the HBM budget,
+and over-budget requests are refused."""
+import numpy as np
+import pytest
+
+import _cases
+import oracle
+from cyclops_amd import data as D
+from cyclops_amd.control import create_control, create_prior
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def E():
+    from cyclops_amd import engine
+    assert engine.device_count() > 0, "no HIP device"
+    return engine
+
+
+def _mix():
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    d0, _ = D.make_config(0, scale=0.5)   # 50,000 rows: a multi-workgroup persistent launch
+    return [
+        (d0, create_prior("laplace", 0.1), ctl),
+        (_cases.logistic(K=2000, seed=11), create_prior("laplace", 0.5), ctl),
+        (_cases.poisson(K=3000, seed=12), create_prior("normal", 1.0), ctl),
+        (_cases.cox(K=1500, seed=13), create_prior("laplace", 0.5), ctl),
+        (_cases.clr(n_strata=300, seed=14), create_prior("laplace", 1.0), ctl),
+        (_cases.logistic(K=800, seed=15), create_prior("normal", 2.0), ctl),
+    ]
+
+
+def _overlap(stats):
+    iv = sorted((s["start"], s["end"]) for s in stats)
+    return any(iv[i + 1][0] < iv[i][1] for i in range(len(iv) - 1))
+
+
+def test_results_same_alone_and_in_mix(E):
+    mix = _mix()
+    alone = []
+    srv = E.Server(slots=1)
+    for i, (d, p, c) in enumerate(mix):
+        print(f"[case] server alone request {i} model={d.model_type} K={d.K}", flush=True)
+        alone.append(srv.poll(srv.submit(d, p, c), wait=True))
+    srv.stop()
+
+    print("[case] server mix of 6 requests, 4 slots", flush=True)
+    srv = E.Server(slots=4)
+    ids = [srv.submit(d, p, c) for d, p, c in mix]
+    together = [srv.poll(i, wait=True) for i in ids]
+    srv.stop()
+
+    for i, ((b1, r1, _), (b2, r2, _)) in enumerate(zip(alone, together)):
+        assert np.array_equal(b1, b2), i
+        assert r1["iterations"] == r2["iterations"] and r1["log_likelihood"] == r2["log_likelihood"]
+    # the mix ran concurrently on the device
+    assert _overlap([t[2] for t in together])
+    # and the results are the oracle's
+    for (d, p, c), (b, r, _) in zip(mix[1:4], together[1:4]):
+        bo, ro = oracle.fit(d, p, c)
+        assert r["iterations"] == ro["iterations"]
+        assert np.max(np.abs(b - bo)) <= 1e-6 * max(np.max(np.abs(bo)), 1e-12)
+
+
+def test_budget_admits_one_at_a_time(E):
+    mix = _mix()[1:5]
+    srv = E.Server(slots=4)
+    foot = [srv.poll(srv.submit(d, p, c), wait=True)[2]["footprint_bytes"] for d, p, c in mix]
+    srv.stop()
+    print(f"[case] server budget {1.5 * max(foot):.0f} bytes for footprints {foot}", flush=True)
+    srv = E.Server(slots=4, hbm_budget_bytes=1.5 * max(foot))
+    ids = [srv.submit(d, p, c) for d, p, c in mix]
+    stats = [srv.poll(i, wait=True)[2] for i in ids]
+    srv.stop()
+    assert not _overlap(stats)   # any two together exceed the budget
+    assert all(s["start"] >= s["submit"] for s in stats)
+
+
+def test_over_budget_request_is_refused(E):
+    srv = E.Server(slots=2, hbm_budget_bytes=1e4)
+    with pytest.raises(E.CyclopsError):
+        srv.submit(_cases.logistic(K=500, seed=3))
+    with pytest.raises(E.CyclopsError):
+        srv.poll(12345)
+    srv.stop()
+
+
+def test_poll_without_wait(E):
+    srv = E.Server(slots=1)
+    rid = srv.submit(_cases.logistic(K=500, seed=4), create_prior("laplace", 1.0), create_control())
+    out = srv.poll(rid, wait=False)
+    assert out is None or out[1]["return_flag"] == 0
+    b, r, st = srv.poll(rid, wait=True)
+    assert r["return_flag"] == 0 and st["end"] >= st["start"] >= st["submit"]
+    srv.stop()
