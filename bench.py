@@ -44,11 +44,21 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--workload", choices=["poisson", "serving"], default="poisson",
+                    help="poisson: the batched-fit headline line (default); serving: the request-mix server")
+    ap.add_argument("--requests", type=int, default=40, help="serving: requests in the mix")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--max-rows", type=int, default=100_000, help="serving: largest request (rows)")
+    ap.add_argument("--max-cols", type=int, default=10_000, help="serving: largest request (columns)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent fits")
+    ap.add_argument("--plain-requests", type=int, default=4, help="serving: requests timed on the plain queue")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serving":
+        return run_serving(args)
 
     os.environ.setdefault("CCD_PROGRESS", "1")   # one line per persistent launch
     from cyclops_amd import _abi
@@ -192,6 +202,110 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _serve(E, reqs, arrivals, slots, mode, budget):
+    """Submit reqs at the given arrival offsets (seconds) to a fresh server; returns
+    per-request (submit, start, end) seconds on the server's clock and the results."""
+    import threading
+    from cyclops_amd.control import create_control, create_prior
+    srv = E.Server(slots=slots, hbm_budget_bytes=budget, mode=mode)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-6)
+    ids = [None] * len(reqs)
+    out = [None] * len(reqs)
+
+    def collect():
+        for i in range(len(reqs)):
+            while ids[i] is None:
+                time.sleep(0.001)
+            out[i] = srv.poll(ids[i], wait=True)
+            s = out[i][2]
+            log(f"{mode} request {i} ({reqs[i]['model']} K={reqs[i]['data'].K} J={reqs[i]['data'].J}): "
+                f"{out[i][1]['iterations']} cycles, latency {s['end'] - s['submit']:.2f}s")
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids[i] = srv.submit(r["data"], create_prior(r["prior"], r["variance"]), ctl, precision=r["precision"])
+    th.join()
+    srv.stop()
+    return out
+
+
+def run_serving(args):
+    """Serving mix: LR / PR / Cox requests (0.5 / 0.3 / 0.2) of log-uniform size, each
+    generated from its own seed, arriving as a Poisson process; the continuous-batching
+    server (resident fits, `slots` concurrent, 200 GB HBM budget) against the plain
+    per-fit queue (one fit at a time through the per-coordinate entry points)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from cyclops_amd import _abi
+    from cyclops_amd import data as D
+    from cyclops_amd import engine as E
+
+    if E.device_count() < 1:
+        raise SystemExit("no HIP device")
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        reqs = list(ex.map(lambda r: D.make_request(r, max_rows=args.max_rows, max_cols=args.max_cols),
+                           range(args.requests)))
+    for r in reqs:
+        r["precision"] = _abi.FP32
+    log(f"generated {len(reqs)} requests in {time.perf_counter() - t0:.1f}s "
+        f"(nnz {min(r['data'].nnz for r in reqs)}..{max(r['data'].nnz for r in reqs)})")
+    rng = np.random.default_rng(4242)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(reqs)))
+    arrivals -= arrivals[0]
+    budget = 200e9
+
+    out = _serve(E, reqs, arrivals, args.slots, "resident", budget)
+    st = [o[2] for o in out]
+    lat = np.array([s["end"] - s["submit"] for s in st])
+    span = max(s["end"] for s in st) - min(s["submit"] for s in st)
+    flags = sorted({o[1]["return_flag"] for o in out})
+    log(f"served {len(reqs)} requests in {span:.2f}s: p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
+
+    plain = None
+    m = min(args.plain_requests, len(reqs))
+    if m > 0:
+        # the same first m requests, arriving at the same times, on the plain queue
+        outp = _serve(E, reqs[:m], arrivals[:m], 1, "plain", budget)
+        stp = [o[2] for o in outp]
+        latp = np.array([s["end"] - s["submit"] for s in stp])
+        spanp = max(s["end"] for s in stp) - min(s["submit"] for s in stp)
+        diff = max(float(np.max(np.abs(a[0] - b[0])) / max(np.max(np.abs(b[0])), 1e-12))
+                   for a, b in zip(out[:m], outp))
+        svc = sum(s["end"] - s["start"] for s in stp)
+        svc_res = sum(s["end"] - s["start"] for s in st[:m])
+        plain = {"requests": m, "fits_per_sec": m / spanp, "p50_s": float(np.percentile(latp, 50)),
+                 "p99_s": float(np.percentile(latp, 99)), "service_seconds": svc,
+                 "served_service_seconds_same_requests": svc_res,
+                 "max_coef_rel_diff_vs_served": diff}
+        log(f"plain queue, first {m} requests: {m / spanp:.3f} fits/s, service {svc:.1f}s vs {svc_res:.1f}s served")
+    res = {
+        "metric": "fits/sec + p50/p99 fit latency vs plain per-fit GPU path at matched beta tol",
+        "value": len(reqs) / span,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": span * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk (f64 prior step, reductions and convergence test)",
+        "data": "synthetic (one seeded generator per request, no dataset files)",
+        "config": {"workload": f"serving_mix_{len(reqs)}req_{args.rate:g}rps_rows1e4-{args.max_rows}_cols1e3-{args.max_cols}",
+                   "models": {k: sum(r["model"] == k for r in reqs) for k in ("lr", "pr", "cox")},
+                   "slots": args.slots, "hbm_budget_bytes": budget, "return_flags": flags},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3,
+                       "note": "arrival to result, server clock"},
+        "plain_queue": plain,
+    }
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -117,4 +117,18 @@ double priorLogDensity(int type, double variance, double beta);
 // Message returned by ccd_last_error() on the calling thread.
 void setLastError(const std::string& msg);
 
+// Device memory arena of the calling thread. While one is installed, engines
+// created on this thread carve their device buffers from it (frees are no-ops; the
+// owner resets `used` when the engines are gone), run on its stream and use its
+// pinned host buffer, so they call no hipMalloc / hipFree / hipHostMalloc, which
+// synchronise the whole device and would wait for other threads' kernels.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    hipStream_t stream = nullptr;
+    double* host = nullptr;   // >= 64 doubles, pinned
+};
+void setThreadArena(Arena* a);   // nullptr: plain hipMalloc / hipFree
+Arena* threadArena();
+
 }  // namespace ccd

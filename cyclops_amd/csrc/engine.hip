@@ -20,17 +20,35 @@ namespace ccd {
 // ---------------------------------------------------------------------------
 // allocation helpers
 // ---------------------------------------------------------------------------
+static thread_local Arena* g_arena = nullptr;
+void setThreadArena(Arena* a) { g_arena = a; }
+Arena* threadArena() { return g_arena; }
+
+static bool inArena(const void* p) {
+    const char* c = static_cast<const char*>(p);
+    return g_arena && c >= g_arena->base && c < g_arena->base + g_arena->cap;
+}
+
 template <typename T>
 static T* dalloc(size_t n, double& bytes) {
     T* p = nullptr;
     const size_t sz = std::max<size_t>(n, 1) * sizeof(T);
+    if (g_arena) {
+        const size_t off = (g_arena->used + 255) & ~size_t(255);
+        if (off + sz <= g_arena->cap) {
+            p = reinterpret_cast<T*>(g_arena->base + off);
+            g_arena->used = off + sz;
+            bytes += (double)sz;
+            return p;
+        }
+    }
     CCD_HIP(hipMalloc(reinterpret_cast<void**>(&p), sz));
     bytes += (double)sz;
     return p;
 }
 template <typename T>
 static void dfree(T*& p) {
-    if (p) (void)hipFree(p);
+    if (p && !inArena(p)) (void)hipFree(p);
     p = nullptr;
 }
 template <typename T>
@@ -140,7 +158,9 @@ static std::shared_ptr<DevData<Real>> makeDevData(const ccd_data_desc& d, int de
     }
     CCD_HIP(hipSetDevice(device));
     hipStream_t s;
-    CCD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const bool ownStream = g_arena == nullptr;
+    if (ownStream) CCD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    else s = g_arena->stream;
     double& B = dd->bytes;
     dd->dRows = dalloc<int32_t>(dd->nnz, B);
     h2d(dd->dRows, d.rows, dd->nnz, s);
@@ -177,7 +197,7 @@ static std::shared_ptr<DevData<Real>> makeDevData(const ccd_data_desc& d, int de
         h2d(dd->dStratStart, dd->stratStart.data(), dd->nStrata + 1, s);
     }
     CCD_HIP(hipStreamSynchronize(s));
-    CCD_HIP(hipStreamDestroy(s));
+    if (ownStream) CCD_HIP(hipStreamDestroy(s));
     return dd;
 }
 
@@ -206,7 +226,9 @@ class EngineT : public EngineBase {
 public:
     EngineT(std::shared_ptr<DevData<Real>> data) : D(std::move(data)) {
         CCD_HIP(hipSetDevice(D->device));
-        CCD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        arena = g_arena;
+        if (arena) s = arena->stream;
+        else CCD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         CCD_HIP(hipEventCreate(&ev0));
         CCD_HIP(hipEventCreate(&ev1));
         const int64_t K = D->K;
@@ -233,7 +255,8 @@ public:
         parts = dalloc<Partial>(MAX_RED_BLOCKS, bytes);
         counter = dalloc<unsigned>(1, bytes);
         dOut = dalloc<double>(8, bytes);
-        CCD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hOut), 8 * sizeof(double), hipHostMallocDefault));
+        if (arena) hOut = arena->host;
+        else CCD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hOut), 8 * sizeof(double), hipHostMallocDefault));
         CCD_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned), s));
         CCD_HIP(hipMemsetAsync(xb, 0, K * sizeof(Real), s));
         CCD_HIP(hipMemsetAsync(xbSave, 0, K * sizeof(Real), s));
@@ -250,10 +273,10 @@ public:
         dfree(denom); dfree(nW); dfree(R1); dfree(R2); dfree(Dacc);
         dfree(pidArr); dfree(pstart); dfree(pseg); dfree(scanParts);
         dfree(XjY); dfree(parts); dfree(counter); dfree(dOut);
-        if (hOut) (void)hipHostFree(hOut);
+        if (hOut && !arena) (void)hipHostFree(hOut);
         (void)hipEventDestroy(ev0);
         (void)hipEventDestroy(ev1);
-        (void)hipStreamDestroy(s);
+        if (!arena) (void)hipStreamDestroy(s);
     }
 
     int model() const override { return D->model; }
@@ -650,6 +673,7 @@ public:
     double fixedTerm = 0.0;
     // resident-fit workspace (resident.hip)
     void* res = nullptr;
+    Arena* arena = nullptr;   // the creating thread's arena, if any (its stream and host buffer)
 };
 
 }  // namespace ccd

@@ -88,8 +88,23 @@ struct ccd_server {
         return nullptr;
     }
 
-    void run(Request& r) {
+    // Each slot allocates from its own device arena (grown only when a request needs
+    // more than it holds), so a request starting or finishing never calls the
+    // device-synchronising hipMalloc / hipFree while other slots' kernels run.
+    static void reserve(ccd::Arena& a, double bytes) {
+        const size_t need = (size_t)(bytes * 1.25) + (size_t(64) << 20);
+        if (a.cap >= need) return;
+        if (a.base) CCD_HIP(hipFree(a.base));
+        a.base = nullptr;
+        a.cap = 0;
+        CCD_HIP(hipMalloc(reinterpret_cast<void**>(&a.base), need));
+        a.cap = need;
+    }
+
+    void run(Request& r, ccd::Arena& arena, bool useArena) {
         CCD_HIP(hipSetDevice(cfg.device));
+        if (useArena) reserve(arena, r.footprint);
+        arena.used = 0;
         ccd_handle h = nullptr;
         int rc = ccd_create(&r.desc, cfg.device, &h);
         if (rc == CCD_OK) {
@@ -101,10 +116,30 @@ struct ccd_server {
         }
         if (rc != CCD_OK) r.error = ccd_last_error();
         if (h) ccd_destroy(h);
+        arena.used = 0;
         r.rc = rc;
     }
 
     void worker() {
+        ccd::Arena arena;
+        bool ready = true;
+        try {
+            CCD_HIP(hipSetDevice(cfg.device));
+            CCD_HIP(hipStreamCreateWithFlags(&arena.stream, hipStreamNonBlocking));
+            CCD_HIP(hipHostMalloc(reinterpret_cast<void**>(&arena.host), 64 * sizeof(double), hipHostMallocDefault));
+            ccd::setThreadArena(&arena);
+        } catch (const std::exception&) {
+            ready = false;   // no arena: this slot allocates per request
+            ccd::setThreadArena(nullptr);
+        }
+        loop(arena, ready);
+        ccd::setThreadArena(nullptr);
+        if (arena.base) (void)hipFree(arena.base);
+        if (arena.host) (void)hipHostFree(arena.host);
+        if (arena.stream) (void)hipStreamDestroy(arena.stream);
+    }
+
+    void loop(ccd::Arena& arena, bool useArena) {
         for (;;) {
             std::shared_ptr<Request> r;
             {
@@ -117,7 +152,7 @@ struct ccd_server {
                 r->stats.start_time = now();
             }
             try {
-                run(*r);
+                run(*r, arena, useArena);
             } catch (const std::exception& ex) {
                 r->rc = CCD_ERR_HIP;
                 r->error = ex.what();
