@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--max-rows", type=int, default=100_000, help="serving: largest request (rows)")
     ap.add_argument("--max-cols", type=int, default=10_000, help="serving: largest request (columns)")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent fits")
+    ap.add_argument("--slots", type=int, default=16, help="serving: concurrent fits")
     ap.add_argument("--plain-requests", type=int, default=4, help="serving: requests timed on the plain queue")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -246,6 +246,9 @@ def run_serving(args):
     from cyclops_amd import data as D
     from cyclops_amd import engine as E
 
+    # one hardware queue per slot, so every slot's persistent kernel can run at once
+    # (set before the first HIP call of the process)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, args.slots)))
     if E.device_count() < 1:
         raise SystemExit("no HIP device")
     t0 = time.perf_counter()

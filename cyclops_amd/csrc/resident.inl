@@ -186,15 +186,16 @@ void EngineT<Real>::releaseResident() {
 // the per-coordinate grid exchange, never more workgroups than compute units.
 // One or two fits are bound by the latency of the per-coordinate exchange, which
 // grows with the number of workgroups; batches are bound by bandwidth and use the
-// whole chip.
-static int chooseWorkgroups(int64_t K, int B, int device) {
+// whole chip. Grouped models (CLR, Cox) also walk their per-pid state after every
+// update, so they take 4x more workgroups.
+static int chooseWorkgroups(int64_t K, int B, int device, bool grouped = false) {
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* e = std::getenv("CCD_RESIDENT_WG")) {
         const int v = std::atoi(e);
         if (v > 0) return std::min(v, std::min(cus, res::MAXG));
     }
-    const int64_t rowsPerWG = B <= 2 ? 16384 : 4096;
+    const int64_t rowsPerWG = (B <= 2 && !grouped) ? 16384 : 4096;
     int64_t g = (K + rowsPerWG - 1) / rowsPerWG;
     if (g < 1) g = 1;
     if (g > cus) g = cus;
@@ -211,13 +212,13 @@ struct Layout {
 };
 
 // CCD_BATCH_LAYOUT="Bt,T" overrides the team shape of batches (B > 2).
-static Layout chooseLayout(int64_t K, int B, int device) {
+static Layout chooseLayout(int64_t K, int B, int device, bool grouped = false) {
     Layout L;
     int bt = 0, t = 0;
     if (const char* e = std::getenv("CCD_BATCH_LAYOUT")) std::sscanf(e, "%d,%d", &bt, &t);
     if (B <= 2) {   // one or two fits: the whole chip in lockstep
         L.Bt = B;
-        L.T = chooseWorkgroups(K, B, device);
+        L.T = chooseWorkgroups(K, B, device, grouped);
     } else if (bt > 0 && t > 0) {
         L.Bt = std::min(bt, 256);
         L.T = std::min(t, res::MAXG);
@@ -282,7 +283,7 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
         for (const Layout& q : out) if (q.Bt == L.Bt && q.T == L.T) return;
         out.push_back(L);
     };
-    if (B <= 256) add(B, chooseWorkgroups(K, B, device));
+    if (B <= 256) add(B, chooseWorkgroups(K, B, device, model == M_CLR || model == M_COX));
     for (int bt : {4, 16, 64}) {
         if (bt >= B) continue;
         for (int t = 8; t <= 128; t *= 2)
@@ -625,7 +626,8 @@ void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_res
         double ms = 0, bytes = 0, steps = 0;
         int nl = 0;
         runResident<Real>(*this, a, 1, &var, nullptr, nullptr, 0, a.weights, &initBound, b.data(), &flag, &it, &ll,
-                          ms, bytes, steps, nl, chooseLayout(D->K, 1, D->device));
+                          ms, bytes, steps, nl,
+                          chooseLayout(D->K, 1, D->device, D->model == M_CLR || D->model == M_COX));
         ++count;
         if (timing) {
             counters.deviceMs += ms;
@@ -725,11 +727,11 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     while (!pending.empty()) {
         std::vector<int> retry;
         const bool whole = tunedOk && (int)pending.size() == n;
-        const Layout L0 = whole ? LT : chooseLayout(D->K, (int)pending.size(), D->device);
+        const Layout L0 = whole ? LT : chooseLayout(D->K, (int)pending.size(), D->device, grouped);
         const size_t chunk = whole ? (size_t)n : (size_t)chunkFits(L0, D->K, nGroups, grouped, (int)sizeof(Real));
         for (size_t c0 = 0; c0 < pending.size(); c0 += chunk) {
             const int B = (int)std::min(chunk, pending.size() - c0);
-            const Layout L = whole ? LT : chooseLayout(D->K, B, D->device);
+            const Layout L = whole ? LT : chooseLayout(D->K, B, D->device, grouped);
             std::vector<double> v(B), bd(B), bb((size_t)B * J), l(B);
             std::vector<int32_t> h(B);
             std::vector<int> fl(B), it(B);

@@ -771,6 +771,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 agg = z;
+#pragma unroll 8
         for (int64_t p = p0; p < p1; ++p)
             agg = segCombine(agg, Seg2{P.denom[p * BP + f], 0.0, (p > 0 && P.pseg[p] != P.pseg[p - 1]) ? 1 : 0, 0});
         s.scan[sg * BP + f] = agg;
@@ -804,6 +805,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 run = segCombine(Seg2{s.carry[f], 0.0, 0, 0}, s.scan[sg * BP + f]);
+#pragma unroll 8
         for (int64_t p = p0; p < p1; ++p) {
             run = segCombine(run, Seg2{P.denom[p * BP + f], 0.0, (p > 0 && P.pseg[p] != P.pseg[p - 1]) ? 1 : 0, 0});
             P.Dacc[p * BP + f] = run.a;
@@ -821,6 +823,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 agg = z;
+#pragma unroll 8
         for (int64_t p = p1 - 1; p >= p0; --p) agg = segCombine(agg, bval(p, f));
         s.scan[sg * BP + f] = agg;
     }
@@ -854,6 +857,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 run = segCombine(Seg2{s.carry[f], s.carry2[f], 0, 0}, s.scan[sg * BP + f]);
+#pragma unroll 8
         for (int64_t p = p1 - 1; p >= p0; --p) {
             run = segCombine(run, bval(p, f));
             P.R1[p * BP + f] = run.a;

@@ -1,11 +1,12 @@
 // server.hip -- the fit-serving loop behind ccd_server_* (include/cyclops_amd.h).
 //
 // Continuous admission: `slots` worker threads each own a HIP stream (one engine
-// per request). A request is admitted, first-fit in arrival order, as soon as a
-// slot is free, its HBM footprint fits what the running requests leave of the
-// budget, and (resident mode) its persistent launch fits what they leave of the
-// device's co-resident workgroups -- so concurrent persistent kernels can always
-// all be resident and never wait on each other. Admitted fits run concurrently on
+// per request). The HBM budget is reserved once as a device region; a request is
+// admitted, first-fit in arrival order, as soon as a slot is free, a block of the
+// region holds its footprint, and (resident mode) its persistent launch fits what
+// the running requests leave of the device's co-resident workgroups -- so
+// concurrent persistent kernels can always all be resident and never wait on each
+// other. Admitted fits run concurrently on
 // the device. A request's kernel shape depends only on its own data, so its result
 // is the same alone and in any mix.
 //
@@ -42,6 +43,7 @@ struct Request {
     // admission
     double footprint = 0.0;
     double capFraction = 0.0;
+    size_t blockOff = 0, blockSize = 0;
     // outcome
     ReqState state = QUEUED;
     int rc = CCD_OK;
@@ -56,6 +58,44 @@ void copyIn(std::vector<T>& dst, const T* src, size_t n) {
     else dst.clear();
 }
 
+// First-fit allocator of blocks inside one device region reserved at server start.
+struct Region {
+    char* base = nullptr;
+    size_t cap = 0;
+    std::map<size_t, size_t> holes;   // offset -> size, disjoint, coalesced
+
+    bool alloc(size_t n, size_t& off) {
+        for (auto it = holes.begin(); it != holes.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            const size_t rest = it->second - n;
+            holes.erase(it);
+            if (rest) holes[off + n] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        auto it = holes.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != holes.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            holes.erase(nx);
+        }
+        if (it != holes.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                holes.erase(it);
+            }
+        }
+    }
+};
+
+// Device bytes a request's engine may carve from its block: the admission estimate
+// with a margin (an engine that needs more allocates the rest with hipMalloc).
+size_t blockBytes(double footprint) { return (((size_t)(footprint * 1.25) + (size_t(1) << 20)) + 255) & ~size_t(255); }
+
 }  // namespace
 
 struct ccd_server {
@@ -66,7 +106,8 @@ struct ccd_server {
     std::condition_variable done;   // a request finished
     std::deque<std::shared_ptr<Request>> queue;
     std::map<int64_t, std::shared_ptr<Request>> all;
-    double usedBytes = 0.0, usedCap = 0.0;
+    double usedCap = 0.0;
+    Region region;   // all requests' device memory
     int64_t nextId = 1;
     bool stopping = false;
     std::vector<std::thread> workers;
@@ -79,8 +120,10 @@ struct ccd_server {
     std::shared_ptr<Request> pick() {
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             const auto& r = *it;
-            if (usedBytes + r->footprint > cfg.hbm_budget_bytes) continue;
             if (usedCap + r->capFraction > 1.0 + 1e-9) continue;
+            size_t off = 0;
+            if (!region.alloc(r->blockSize, off)) continue;
+            r->blockOff = off;
             auto q = r;
             queue.erase(it);
             return q;
@@ -88,23 +131,16 @@ struct ccd_server {
         return nullptr;
     }
 
-    // Each slot allocates from its own device arena (grown only when a request needs
-    // more than it holds), so a request starting or finishing never calls the
-    // device-synchronising hipMalloc / hipFree while other slots' kernels run.
-    static void reserve(ccd::Arena& a, double bytes) {
-        const size_t need = (size_t)(bytes * 1.25) + (size_t(64) << 20);
-        if (a.cap >= need) return;
-        if (a.base) CCD_HIP(hipFree(a.base));
-        a.base = nullptr;
-        a.cap = 0;
-        CCD_HIP(hipMalloc(reinterpret_cast<void**>(&a.base), need));
-        a.cap = need;
-    }
-
+    // A request's engine carves its device memory from the request's block of the
+    // server's region, so starting or finishing a request calls no hipMalloc / hipFree
+    // (which synchronise the whole device and would wait for the other slots' kernels).
     void run(Request& r, ccd::Arena& arena, bool useArena) {
         CCD_HIP(hipSetDevice(cfg.device));
-        if (useArena) reserve(arena, r.footprint);
-        arena.used = 0;
+        if (useArena) {
+            arena.base = region.base + r.blockOff;
+            arena.cap = r.blockSize;
+            arena.used = 0;
+        }
         ccd_handle h = nullptr;
         int rc = ccd_create(&r.desc, cfg.device, &h);
         if (rc == CCD_OK) {
@@ -116,7 +152,8 @@ struct ccd_server {
         }
         if (rc != CCD_OK) r.error = ccd_last_error();
         if (h) ccd_destroy(h);
-        arena.used = 0;
+        arena.base = nullptr;
+        arena.cap = arena.used = 0;
         r.rc = rc;
     }
 
@@ -134,7 +171,6 @@ struct ccd_server {
         }
         loop(arena, ready);
         ccd::setThreadArena(nullptr);
-        if (arena.base) (void)hipFree(arena.base);
         if (arena.host) (void)hipHostFree(arena.host);
         if (arena.stream) (void)hipStreamDestroy(arena.stream);
     }
@@ -146,7 +182,6 @@ struct ccd_server {
                 std::unique_lock<std::mutex> lk(m);
                 work.wait(lk, [&] { return (r = pick()) != nullptr || (stopping && queue.empty()); });
                 if (!r) return;
-                usedBytes += r->footprint;
                 usedCap += r->capFraction;
                 r->state = RUNNING;
                 r->stats.start_time = now();
@@ -159,7 +194,7 @@ struct ccd_server {
             }
             {
                 std::lock_guard<std::mutex> lk(m);
-                usedBytes -= r->footprint;
+                region.release(r->blockOff, r->blockSize);
                 usedCap -= r->capFraction;
                 r->stats.end_time = now();
                 r->state = DONE;
@@ -214,6 +249,23 @@ int ccd_server_start(const ccd_server_config* cfg, ccd_server_handle* out) {
         auto* s = new ccd_server();
         s->cfg = *cfg;
         s->t0 = std::chrono::steady_clock::now();
+        // the admission budget, reserved once (at most 90% of what the device has free)
+        CCD_HIP(hipSetDevice(cfg->device));
+        size_t freeB = 0, totalB = 0;
+        CCD_HIP(hipMemGetInfo(&freeB, &totalB));
+        size_t want = std::min((size_t)cfg->hbm_budget_bytes, (size_t)(0.9 * (double)freeB));
+        want &= ~size_t(255);
+        if (want < (size_t(1) << 20)) {
+            delete s;
+            throw std::invalid_argument("hbm_budget_bytes leaves no room for a request");
+        }
+        if (hipMalloc(reinterpret_cast<void**>(&s->region.base), want) != hipSuccess) {
+            delete s;
+            throw ccd::HipError("could not reserve the serving region of " + std::to_string(want) + " bytes");
+        }
+        s->region.cap = want;
+        s->region.holes[0] = want;
+        s->cfg.hbm_budget_bytes = (double)want;
         for (int i = 0; i < cfg->slots; ++i) s->workers.emplace_back([s] { s->worker(); });
         *out = s;
         return CCD_OK;
@@ -271,8 +323,10 @@ int ccd_server_submit(ccd_server_handle s, const ccd_data_desc* d, const ccd_fit
             r->capFraction = n.capacityFraction;
         }
         r->footprint = bytes;
-        if (bytes > s->cfg.hbm_budget_bytes)
-            throw std::invalid_argument("request needs " + std::to_string(bytes) + " bytes of HBM, over the budget");
+        r->blockSize = blockBytes(bytes);
+        if ((double)r->blockSize > s->cfg.hbm_budget_bytes)
+            throw std::invalid_argument("request needs " + std::to_string(r->blockSize) +
+                                        " bytes of HBM, over the budget of " + std::to_string(s->cfg.hbm_budget_bytes));
         {
             std::lock_guard<std::mutex> lk(s->m);
             if (s->stopping) throw std::runtime_error("server is stopping");
@@ -320,6 +374,7 @@ int ccd_server_stop(ccd_server_handle s) {
         }
         s->work.notify_all();
         for (auto& t : s->workers) t.join();
+        if (s->region.base) (void)hipFree(s->region.base);
         delete s;
         return CCD_OK;
     });

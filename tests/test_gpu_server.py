@@ -40,14 +40,14 @@ def _overlap(stats):
 def test_results_same_alone_and_in_mix(E):
     mix = _mix()
     alone = []
-    srv = E.Server(slots=1)
+    srv = E.Server(slots=1, hbm_budget_bytes=8e9)
     for i, (d, p, c) in enumerate(mix):
         print(f"[case] server alone request {i} model={d.model_type} K={d.K}", flush=True)
         alone.append(srv.poll(srv.submit(d, p, c), wait=True))
     srv.stop()
 
     print("[case] server mix of 6 requests, 4 slots", flush=True)
-    srv = E.Server(slots=4)
+    srv = E.Server(slots=4, hbm_budget_bytes=8e9)
     ids = [srv.submit(d, p, c) for d, p, c in mix]
     together = [srv.poll(i, wait=True) for i in ids]
     srv.stop()
@@ -64,13 +64,19 @@ def test_results_same_alone_and_in_mix(E):
         assert np.max(np.abs(b - bo)) <= 1e-6 * max(np.max(np.abs(bo)), 1e-12)
 
 
+def _block(footprint):   # device block the server reserves for a request
+    return ((int(footprint * 1.25) + (1 << 20)) + 255) & ~255
+
+
 def test_budget_admits_one_at_a_time(E):
     mix = _mix()[1:5]
-    srv = E.Server(slots=4)
-    foot = [srv.poll(srv.submit(d, p, c), wait=True)[2]["footprint_bytes"] for d, p, c in mix]
+    srv = E.Server(slots=4, hbm_budget_bytes=8e9)
+    blocks = sorted(_block(srv.poll(srv.submit(d, p, c), wait=True)[2]["footprint_bytes"]) for d, p, c in mix)
     srv.stop()
-    print(f"[case] server budget {1.5 * max(foot):.0f} bytes for footprints {foot}", flush=True)
-    srv = E.Server(slots=4, hbm_budget_bytes=1.5 * max(foot))
+    assert blocks[-1] < blocks[0] + blocks[1]
+    budget = (blocks[-1] + blocks[0] + blocks[1]) // 2   # holds any one request, no two
+    print(f"[case] server budget {budget} bytes for blocks {blocks}", flush=True)
+    srv = E.Server(slots=4, hbm_budget_bytes=budget)
     ids = [srv.submit(d, p, c) for d, p, c in mix]
     stats = [srv.poll(i, wait=True)[2] for i in ids]
     srv.stop()
@@ -79,16 +85,18 @@ def test_budget_admits_one_at_a_time(E):
 
 
 def test_over_budget_request_is_refused(E):
-    srv = E.Server(slots=2, hbm_budget_bytes=1e4)
     with pytest.raises(E.CyclopsError):
-        srv.submit(_cases.logistic(K=500, seed=3))
+        E.Server(slots=2, hbm_budget_bytes=1e4)   # not even one request fits
+    srv = E.Server(slots=2, hbm_budget_bytes=4e6)
+    with pytest.raises(E.CyclopsError):
+        srv.submit(_cases.logistic(K=200_000, seed=3))
     with pytest.raises(E.CyclopsError):
         srv.poll(12345)
     srv.stop()
 
 
 def test_poll_without_wait(E):
-    srv = E.Server(slots=1)
+    srv = E.Server(slots=1, hbm_budget_bytes=8e9)
     rid = srv.submit(_cases.logistic(K=500, seed=4), create_prior("laplace", 1.0), create_control())
     out = srv.poll(rid, wait=False)
     assert out is None or out[1]["return_flag"] == 0
