@@ -286,7 +286,7 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
     if (B <= 256) add(B, chooseWorkgroups(K, B, device, model == M_CLR || model == M_COX));
     for (int bt : {4, 16, 64}) {
         if (bt >= B) continue;
-        for (int t = 8; t <= 128; t *= 2)
+        for (int t : {8, 16, 24, 32, 48, 64, 96, 128})
             if (K / t >= 2048) add(bt, t);
     }
     if (K <= 200000) { add(1, 1); add(4, 1); }
@@ -429,7 +429,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         W.steps = dalloc<unsigned long long>(1, Bb);
         W.stats = dalloc<double>(8, Bb);
         W.kw = dalloc<Real>(K, Bb);
-        W.teams = dalloc<res::Params<Real>>(nTeams, Bb);
+        W.teams = dalloc<res::Params<Real>>((size_t)nTeams * G, Bb);   // one block per workgroup
     }
     // ---- inputs ---------------------------------------------------------------
     std::vector<unsigned char> excl(J, 0), fix(J, 0);
@@ -554,7 +554,17 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         Q.slots = W.slots + (size_t)t * 2 * G * BP * res::SLOTW;
         Q.bcast = W.bcast + (size_t)t * 2 * BP * 4;
     }
-    h2d(W.teams, teamP.data(), nTeams, s);
+    // the kernel reads workgroup i's block: its team's parameters with w = i mod G
+    std::vector<res::Params<Real>> blockP((size_t)nTeams * G);
+    auto fillBlocks = [&]() {
+        for (int t = 0; t < nTeams; ++t)
+            for (int w = 0; w < G; ++w) {
+                blockP[(size_t)t * G + w] = teamP[t];
+                blockP[(size_t)t * G + w].w = w;
+            }
+        h2d(W.teams, blockP.data(), blockP.size(), s);
+    };
+    fillBlocks();
     // ---- launches: each runs up to `budget` cycles, state stays in HBM -----------
     int budget = 32;
     if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) budget = std::max(1, std::atoi(e));
@@ -569,7 +579,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         h2d(W.allDone, &one, 1, s);
         if (launches <= 1) {   // the first launch starts the fits, later ones resume them
             for (auto& Q : teamP) { Q.resume = launches; Q.cycleBudget = budget; }
-            h2d(W.teams, teamP.data(), nTeams, s);
+            fillBlocks();
         }
         switch (m) {
         case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, W.teams, nTeams, G, s); break;

@@ -1009,8 +1009,9 @@ __device__ __forceinline__ Meta loadMeta(const Params<Real>& P, int j, int w) {
 template <typename Real, int MODEL, int BP>
 __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restrict__ teams, int T) {
     using GE = Geo<BP>;
-    Params<Real> P = teams[blockIdx.x / T];
-    P.w = (int)(blockIdx.x % T);
+    // one parameter block per workgroup (w filled in by the host): fields are read
+    // from the scalar cache where used instead of being held in registers
+    const Params<Real>& P = teams[blockIdx.x];
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     __shared__ Shared<BP> s;
     const int w = P.w;

@@ -230,7 +230,22 @@ class Engine:
         check(lib().ccd_fit_batch(self.h, C.byref(hold.args), C.byref(b), _d(betas), results))
         return betas, [r.as_dict() for r in results]
 
-    def cross_validate(self, prior, control, selector=None):
+    def cross_validate(self, prior, control, selector=None, refit_fp64=True):
+        """Grid-search CV (fold x grid fits batched on this engine) and the final fit at
+        the best variance. With FP32 bulk and refit_fp64, the final fit runs on an FP64
+        engine over the same data."""
+        out = self._cross_validate(prior, control, selector)
+        if refit_fp64 and self.precision == _abi.FP32:
+            e64 = Engine(self.data, _abi.FP64)
+            try:
+                p = create_prior(prior.prior_type, out["best_variance"], exclude=prior.exclude)
+                out["beta"], out["result"] = e64.fit(p, control, mode="resident")
+                out["refit_precision"] = _abi.FP64
+            finally:
+                e64.close()
+        return out
+
+    def _cross_validate(self, prior, control, selector=None):
         if selector is None:
             # R/ModelFit.R:174-188: "auto" -> byPid for conditional models, byRow otherwise
             selector = _abi.SELECT_BY_PID if self.data.model_type in (_abi.CONDITIONAL_LOGISTIC,) else \

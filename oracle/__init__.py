@@ -161,9 +161,17 @@ def fit(data, prior=None, control=None, weights=None, fixed=None, beta0=None):
     return beta, res.as_dict()
 
 
-def time_cycles(data, prior, control, cycles, beta0=None):
+def prior_delta(prior_type, variance, g, h, beta):
+    """The prior's coordinate step (CovariatePrior.h getDelta) for gradient g, Hessian h."""
+    L = lib()
+    L.orc_prior_delta.restype = C.c_double
+    L.orc_prior_delta.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    return L.orc_prior_delta(_abi.PRIOR_NAMES[prior_type], variance, g, h, beta)
+
+
+def time_cycles(data, prior, control, cycles, beta0=None, weights=None):
     """Seconds for exactly `cycles` CCD cycles on one CPU core."""
-    hold = _args(data, prior, control)
+    hold = _args(data, prior, control, weights=weights)
     eng = OracleEngine(data)
     beta = np.zeros(data.J) if beta0 is None else np.array(beta0, dtype=np.float64)
     s = lib().orc_time_cycles(eng.h, C.byref(hold.args), _d(beta), int(cycles))

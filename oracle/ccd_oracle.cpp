@@ -782,6 +782,13 @@ int orc_fit(void* hv, const ccd_fit_args* a, double* beta, ccd_fit_result* r) {
 
 /* Per-cycle timing: run exactly `cycles` CCD cycles (no convergence test) from
  * beta and report seconds; used to extrapolate CPU fits that take too long. */
+double orc_prior_delta(int type, double variance, double g, double h, double beta) {
+    orc::Prior p;
+    p.type = type;
+    p.variance = variance;
+    return p.delta(g, h, beta);
+}
+
 double orc_time_cycles(void* hv, const ccd_fit_args* a, double* beta, int cycles) {
     auto* h = static_cast<orc_handle*>(hv);
     orc::CCD ccd(h->data, *h->ms);

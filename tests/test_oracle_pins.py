@@ -390,3 +390,41 @@ def test_fold_assignment_product_matches_oracle():
         a = engine.assign_folds(150, pid, 10, sel, 42)
         b = oracle.assign_folds(150, pid, 10, sel, 42)
         assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# Prior step: the minimiser of the penalised quadratic model of the coordinate,
+# g d + h d^2 / 2 + penalty(beta + d), found by brute force. The Laplace step may
+# not carry beta across zero in one step (it stops at zero), so for beta != 0 the
+# minimum is taken over the half-line that keeps the sign of beta.
+# ---------------------------------------------------------------------------
+def _argmin(f, lo, hi, n=200001):
+    x = np.linspace(lo, hi, n)
+    return x[np.argmin(f(x))], (hi - lo) / (n - 1)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_prior_step_minimises_penalised_model(seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(40):
+        g = rng.normal(0, 3)
+        h = rng.uniform(0.2, 4)
+        var = float(np.exp(rng.uniform(np.log(0.05), np.log(5))))
+        lam = np.sqrt(2 / var)
+        beta = 0.0 if rng.random() < 0.4 else rng.normal(0, 1.5)
+        q = lambda d: g * d + 0.5 * h * d * d
+        R = 2 * (abs(g) + lam + abs(beta) / var) / h + 2 * abs(beta) + 1   # holds every minimiser
+        # none
+        d0, step = _argmin(q, -R, R)
+        assert abs(oracle.prior_delta("none", var, g, h, beta) - d0) <= 2 * step
+        # normal
+        dn, step = _argmin(lambda d: q(d) + (beta + d) ** 2 / (2 * var), -R, R)
+        assert abs(oracle.prior_delta("normal", var, g, h, beta) - dn) <= 2 * step
+        # laplace
+        if beta == 0:
+            dl, step = _argmin(lambda d: q(d) + lam * np.abs(d), -R, R)
+        elif beta > 0:
+            dl, step = _argmin(lambda d: q(d) + lam * (beta + d), -beta, R)
+        else:
+            dl, step = _argmin(lambda d: q(d) - lam * (beta + d), -R, -beta)
+        assert abs(oracle.prior_delta("laplace", var, g, h, beta) - dl) <= 2 * step, (g, h, var, beta)
