@@ -566,21 +566,27 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     };
     fillBlocks();
     // ---- launches: each runs up to `budget` cycles, state stays in HBM -----------
-    int budget = 32;
-    if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) budget = std::max(1, std::atoi(e));
+    // Cycles per launch: CCD_CYCLES_PER_LAUNCH, else adapted so a launch lasts about
+    // 4 s (the host sees progress and errors at launch boundaries): 2 cycles first,
+    // then as many as the measured cycle time allows, at most 32.
+    int budget = 2;
+    bool fixedBudget = false;
+    if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) { budget = std::max(1, std::atoi(e)); fixedBudget = true; }
     const bool progress = std::getenv("CCD_PROGRESS") != nullptr;
     const auto tWall = std::chrono::steady_clock::now();
     CCD_HIP(hipEventRecord(E.ev0, s));
-    int done = 0, launches = 0;
+    int done = 0, launches = 0, cyclesRun = 0, lastBudget = -1;
     const int one = 1;
     while (!done) {
         CCD_HIP(hipMemsetAsync(W.epoch, 0, (size_t)nTeams * G * sizeof(unsigned), s));
         CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
         h2d(W.allDone, &one, 1, s);
-        if (launches <= 1) {   // the first launch starts the fits, later ones resume them
-            for (auto& Q : teamP) { Q.resume = launches; Q.cycleBudget = budget; }
+        if (launches <= 1 || budget != lastBudget) {   // the first launch starts the fits, later ones resume
+            for (auto& Q : teamP) { Q.resume = launches > 0 ? 1 : 0; Q.cycleBudget = budget; }
             fillBlocks();
+            lastBudget = budget;
         }
+        const auto tL = std::chrono::steady_clock::now();
         switch (m) {
         case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, W.teams, nTeams, G, s); break;
         case M_POISSON: launchByBP<Real, M_POISSON>(BP, W.teams, nTeams, G, s); break;
@@ -595,10 +601,15 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         if (hs[0]) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
         done = hs[1];
         ++launches;
+        cyclesRun += budget;
         if (progress)
             std::fprintf(stderr, "[ccd] launch %d (%d teams x %d workgroups, %d fits): %d cycles, %s (%.1fs)\n",
-                         launches, nTeams, G, B, launches * budget, done ? "all fits finished" : "fits still running",
+                         launches, nTeams, G, B, cyclesRun, done ? "all fits finished" : "fits still running",
                          std::chrono::duration<double>(std::chrono::steady_clock::now() - tWall).count());
+        if (!fixedBudget && !done) {
+            const double perCycle = std::chrono::duration<double>(std::chrono::steady_clock::now() - tL).count() / budget;
+            budget = (int)std::max(1.0, std::min(32.0, std::floor(4.0 / std::max(perCycle, 1e-6))));
+        }
     }
     CCD_HIP(hipEventRecord(E.ev1, s));
     CCD_HIP(hipStreamSynchronize(s));
@@ -728,6 +739,7 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
                         std::fprintf(stderr, "[ccd] autotune %d fits: %d fits x %d workgroups per team: %.1f ms per cycle\n",
                                      n, L.Bt, L.T, ms);
                     if (ms < best) { best = ms; LT = L; }
+                    if (best > 2000.0) break;   // every shape is slow here: probing costs more than it saves
                 }
                 W.tuned[n] = {LT.Bt, LT.T, LT.nTeams};
                 tunedOk = true;

@@ -238,12 +238,29 @@ struct Unroll {
     static constexpr int U = Geo<BP>::FPL >= 4 ? 4 : (BP >= 16 ? 16 : 8);
 };
 
+// Lane geometry per model. Wide teams of independent-row models give each lane 4
+// fits of an entry (one 16-byte xb gather per lane), so 4x more entries of a
+// column are in flight per workgroup than with one fit per lane (for 4 fits per
+// team that costs more registers than it gains); grouped models keep Geo<BP>,
+// whose ordered run sums need one fit per lane for BP < 256.
+template <int BP, int MODEL>
+struct GeoM {
+    static constexpr bool WIDE = (MODEL == M_LOGISTIC || MODEL == M_POISSON) && BP >= 16;
+    static constexpr int SUB = WIDE ? (BP / 4 < 64 ? BP / 4 : 64) : Geo<BP>::SUB;
+    static constexpr int NSG = BLOCK / SUB;
+    static constexpr int FPL = BP / SUB;
+};
+template <int BP, int MODEL>
+struct UnrollM {
+    static constexpr int U = GeoM<BP, MODEL>::FPL >= 4 ? 4 : (BP >= 16 ? 16 : 8);
+};
+
 // U consecutive entries of one sub-group's chunk with everything the column pass
 // and the update need; held in registers across the grid exchange when a chunk
 // has at most U entries (the update then only stores).
-template <typename Real, int MODEL, int BP, int U = Unroll<BP>::U>
+template <typename Real, int MODEL, int BP, int U = UnrollM<BP, MODEL>::U>
 struct Batch {
-    using GE = Geo<BP>;
+    using GE = GeoM<BP, MODEL>;
     int kk[U];
     Real xx[U];
     Real xv[U][GE::FPL];
@@ -272,9 +289,9 @@ __device__ __forceinline__ void loadRows(const Params<Real>& P, int j, int64_t e
 
 // Per-fit state of the rows of a batch (xb, weights, risk-set ratios).
 template <typename Real, int MODEL, int BP, int U>
-__device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on)[Geo<BP>::FPL],
+__device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on)[GeoM<BP, MODEL>::FPL],
                                           Batch<Real, MODEL, BP, U>& bt) {
-    using GE = Geo<BP>;
+    using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sl = threadIdx.x % GE::SUB;
 #pragma unroll
@@ -295,16 +312,16 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
 
 template <typename Real, int MODEL, int BP, int FMT, int U>
 __device__ __forceinline__ void loadBatch(const Params<Real>& P, int j, int64_t e, int64_t n1,
-                                          const bool (&on)[Geo<BP>::FPL], Batch<Real, MODEL, BP, U>& bt) {
+                                          const bool (&on)[GeoM<BP, MODEL>::FPL], Batch<Real, MODEL, BP, U>& bt) {
     loadRows<Real, MODEL, BP, FMT, U>(P, j, e, n1, bt);
     loadState<Real, MODEL, BP, U>(P, on, bt);
 }
 
 // Gradient terms of a batch, strictly in entry order.
 template <typename Real, int MODEL, int BP, int U>
-__device__ __forceinline__ void accumBatch(const Batch<Real, MODEL, BP, U>& bt, const bool (&on)[Geo<BP>::FPL],
-                                           Mon (&acc)[Geo<BP>::FPL], int& prevSeg, bool& hasPrev) {
-    using GE = Geo<BP>;
+__device__ __forceinline__ void accumBatch(const Batch<Real, MODEL, BP, U>& bt, const bool (&on)[GeoM<BP, MODEL>::FPL],
+                                           Mon (&acc)[GeoM<BP, MODEL>::FPL], int& prevSeg, bool& hasPrev) {
+    using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -351,8 +368,8 @@ __device__ __forceinline__ void accumBatch(const Batch<Real, MODEL, BP, U>& bt, 
 // Column pass over the sub-group chunk [n0, n1) (run-aligned for grouped models).
 template <typename Real, int MODEL, int BP, int FMT>
 __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n0, int64_t n1, int64_t c0,
-                                         Mon (&acc)[Geo<BP>::FPL], const bool (&on)[Geo<BP>::FPL]) {
-    constexpr int U = Unroll<BP>::U;
+                                         Mon (&acc)[GeoM<BP, MODEL>::FPL], const bool (&on)[GeoM<BP, MODEL>::FPL]) {
+    constexpr int U = UnrollM<BP, MODEL>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     int prevSeg = 0;
     bool hasPrev = false;
@@ -373,8 +390,8 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n
 // pass (fits that moved were active there), per-run denominators as updatePass.
 template <typename Real, int MODEL, int BP, int U>
 __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Real, MODEL, BP, U>& bt,
-                                           const double (&dl)[Geo<BP>::FPL]) {
-    using GE = Geo<BP>;
+                                           const double (&dl)[GeoM<BP, MODEL>::FPL]) {
+    using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sl = threadIdx.x % GE::SUB;
     double acc[GE::FPL];
@@ -422,9 +439,9 @@ __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Re
 
 template <typename Real, int MODEL, int BP, int FMT>
 __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t n0, int64_t n1,
-                                           const double (&dl)[Geo<BP>::FPL]) {
-    using GE = Geo<BP>;
-    constexpr int U = Unroll<BP>::U;
+                                           const double (&dl)[GeoM<BP, MODEL>::FPL]) {
+    using GE = GeoM<BP, MODEL>;
+    constexpr int U = UnrollM<BP, MODEL>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sl = threadIdx.x % GE::SUB;
     double acc[GE::FPL];
@@ -512,9 +529,8 @@ __device__ __forceinline__ void combineSubgroups(Mon* sMon) {
 // Plain sums of (g, h) per fit over the sub-groups of a workgroup: a butterfly
 // over the lanes of a wave that hold the same fit, then the waves in order (a
 // fixed order, so results are reproducible). Result in sMon[f] (g, h).
-template <int BP>
-__device__ __forceinline__ void combinePlain(const Mon (&acc)[Geo<BP>::FPL], Mon* sMon) {
-    using GE = Geo<BP>;
+template <int BP, class GE>
+__device__ __forceinline__ void combinePlain(const Mon (&acc)[GE::FPL], Mon* sMon) {
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int sl = threadIdx.x % GE::SUB;
 #pragma unroll
@@ -879,7 +895,7 @@ enum ObjOp : int { OBJ_LOGLIK = 0, OBJ_GRADIENT = 1, OBJ_LANGE = 2 };
 // This workgroup's share of the objective of every fit, into s.mon[f].g.
 template <typename Real, int MODEL, int BP>
 __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, Shared<BP>& s) {
-    using GE = Geo<BP>;
+    using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
     const int w = P.w;
@@ -934,7 +950,7 @@ __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, 
     Mon m[GE::FPL];
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) m[i] = Mon{acc[i], 0, 0, 0, 0, 0};
-    combinePlain<BP>(m, s.mon);
+    combinePlain<BP, GE>(m, s.mon);
 }
 
 // Per-pid initial state of this workgroup: nE, denominators, CLR risk ratios.
@@ -1008,7 +1024,7 @@ __device__ __forceinline__ Meta loadMeta(const Params<Real>& P, int j, int w) {
 // (its own data and state pointers), exchanging only inside the team.
 template <typename Real, int MODEL, int BP>
 __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restrict__ teams, int T) {
-    using GE = Geo<BP>;
+    using GE = GeoM<BP, MODEL>;
     // one parameter block per workgroup (w filled in by the host): fields are read
     // from the scalar cache where used instead of being held in registers
     const Params<Real>& P = teams[blockIdx.x];
@@ -1228,7 +1244,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     __syncthreads();
                     combineSubgroups<BP>(s.mon);
                 } else {
-                    combinePlain<BP>(acc, s.mon);
+                    combinePlain<BP, GE>(acc, s.mon);
                 }
                 const unsigned long long tB = now();
                 writeSlots<Real, BP>(P, seq, s, GROUPED);

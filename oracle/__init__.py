@@ -169,6 +169,16 @@ def prior_delta(prior_type, variance, g, h, beta):
     return L.orc_prior_delta(_abi.PRIOR_NAMES[prior_type], variance, g, h, beta)
 
 
+def time_coordinates(data, prior, control, n, weights=None):
+    """Seconds for the first n coordinates of the first cycle from a cold start, one core."""
+    hold = _args(data, prior, control, weights=weights)
+    eng = OracleEngine(data)
+    L = lib()
+    L.orc_time_coordinates.restype = C.c_double
+    L.orc_time_coordinates.argtypes = [C.c_void_p, C.POINTER(_abi.FitArgs), C.c_int]
+    return L.orc_time_coordinates(eng.h, C.byref(hold.args), int(n))
+
+
 def time_cycles(data, prior, control, cycles, beta0=None, weights=None):
     """Seconds for exactly `cycles` CCD cycles on one CPU core."""
     hold = _args(data, prior, control, weights=weights)

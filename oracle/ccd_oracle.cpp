@@ -617,6 +617,28 @@ public:
         lastIterations = iteration;
     }
 
+    // Seconds for the first n coordinates of the first cycle (the loop body of
+    // findMode above), for a bounded CPU timing sample.
+    double timeCoordinates(int n) {
+        if (!validWeights) { computeNEvents(); ms.fixedTerms(useCV); validWeights = true; }
+        if (!xBetaKnown) { computeXBeta(); xBetaKnown = true; suffKnown = false; }
+        if (!suffKnown) { ms.remainingStats(useCV); suffKnown = true; }
+        std::fill(hDelta.begin(), hDelta.end(), initialBound);
+        const auto t0 = std::chrono::steady_clock::now();
+        int done = 0;
+        for (int j = 0; j < J && done < n; ++j) {
+            if (fixBeta[j]) continue;
+            double delta = ccdUpdateBeta(j);
+            delta = applyBounds(delta, j);
+            if (delta != 0.0) {
+                beta[j] += delta;
+                ms.updateXBeta(delta, j, useCV);
+            }
+            ++done;
+        }
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
     void update(const ccd_fit_args& a) {   // :586-615
         initialBound = a.initial_bound;
         int count = 0;
@@ -782,6 +804,19 @@ int orc_fit(void* hv, const ccd_fit_args* a, double* beta, ccd_fit_result* r) {
 
 /* Per-cycle timing: run exactly `cycles` CCD cycles (no convergence test) from
  * beta and report seconds; used to extrapolate CPU fits that take too long. */
+double orc_time_coordinates(void* hv, const ccd_fit_args* a, int n) {
+    auto* h = static_cast<orc_handle*>(hv);
+    orc::CCD ccd(h->data, *h->ms);
+    orc::configure(ccd, a);
+    std::vector<double> b(h->data.J, 0.0);
+    if (h->data.hasOffset) b[0] = 1.0;
+    ccd.setBeta(b.data());
+    if (a->weights) ccd.setWeights(a->weights);
+    ccd.initialBound = a->initial_bound;
+    ccd.lazy();   // set-up outside the timed region
+    return ccd.timeCoordinates(n);
+}
+
 double orc_prior_delta(int type, double variance, double g, double h, double beta) {
     orc::Prior p;
     p.type = type;
