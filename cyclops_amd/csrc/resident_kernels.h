@@ -782,14 +782,54 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     const int64_t N = P.N;
     auto segc = [](const Seg2& a, const Seg2& b) { return segCombine(a, b); };
     const Seg2 z{0.0, 0.0, 0, 0};
+    // The pids of a chunk are visited TL at a time: every load of a tile is issued
+    // before the first combine, so a chunk costs one memory latency per TL pids.
+    constexpr int TL = 8;
+    // forward term of pid p: denominator, reset at a new stratum
+    auto fwdTiles = [&](int64_t a, int64_t b, int f, auto visit) {
+        for (int64_t p = a; p < b; p += TL) {
+            double dv[TL];
+            int sv[TL + 1];
+            sv[0] = p > 0 ? P.pseg[p - 1] : 0;
+#pragma unroll
+            for (int t = 0; t < TL; ++t) {
+                const bool ok = p + t < b;
+                dv[t] = ok ? P.denom[(p + t) * BP + f] : 0.0;
+                sv[t + 1] = ok ? P.pseg[p + t] : 0;
+            }
+#pragma unroll
+            for (int t = 0; t < TL; ++t)
+                if (p + t < b) visit(p + t, Seg2{dv[t], 0.0, (p + t > 0 && sv[t + 1] != sv[t]) ? 1 : 0, 0});
+        }
+    };
+    // backward term of pid p: nE/D and nE/D^2, reset after the last pid of a stratum
+    auto bwdTiles = [&](int64_t a, int64_t b, int f, auto visit) {
+        for (int64_t p = b - 1; p >= a; p -= TL) {
+            double nv[TL], dv[TL];
+            int sv[TL + 1];   // sv[t] = pseg[p - t + 1]
+            sv[0] = P.pseg[p + 1];
+#pragma unroll
+            for (int t = 0; t < TL; ++t) {
+                const bool ok = p - t >= a;
+                nv[t] = ok ? P.nW[(p - t) * BP + f] : 0.0;
+                dv[t] = ok ? P.Dacc[(p - t) * BP + f] : 1.0;
+                sv[t + 1] = ok ? P.pseg[p - t] : 0;
+            }
+#pragma unroll
+            for (int t = 0; t < TL; ++t) {
+                const int64_t q = p - t;
+                if (q < a) break;
+                const double r = nv[t] == 0.0 ? 0.0 : nv[t] / dv[t];
+                visit(q, Seg2{r, nv[t] == 0.0 ? 0.0 : r / dv[t], (q == N - 1 || sv[t] != sv[t + 1]) ? 1 : 0, 0});
+            }
+        }
+    };
     // ---- forward: D = segmented prefix of denom
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 agg = z;
-#pragma unroll 8
-        for (int64_t p = p0; p < p1; ++p)
-            agg = segCombine(agg, Seg2{P.denom[p * BP + f], 0.0, (p > 0 && P.pseg[p] != P.pseg[p - 1]) ? 1 : 0, 0});
+        fwdTiles(p0, p1, f, [&](int64_t, const Seg2& v) { agg = segCombine(agg, v); });
         s.scan[sg * BP + f] = agg;
     }
     __syncthreads();
@@ -821,26 +861,18 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 run = segCombine(Seg2{s.carry[f], 0.0, 0, 0}, s.scan[sg * BP + f]);
-#pragma unroll 8
-        for (int64_t p = p0; p < p1; ++p) {
-            run = segCombine(run, Seg2{P.denom[p * BP + f], 0.0, (p > 0 && P.pseg[p] != P.pseg[p - 1]) ? 1 : 0, 0});
+        fwdTiles(p0, p1, f, [&](int64_t p, const Seg2& v) {
+            run = segCombine(run, v);
             P.Dacc[p * BP + f] = run.a;
-        }
+        });
     }
     __syncthreads();
     // ---- backward: R1, R2 = reverse segmented sums of nE/D, nE/D^2
-    auto bval = [&](int64_t p, int f) {
-        const double nw = P.nW[p * BP + f];
-        const double d = P.Dacc[p * BP + f];
-        const double r = nw == 0.0 ? 0.0 : nw / d;
-        return Seg2{r, nw == 0.0 ? 0.0 : r / d, (p == N - 1 || P.pseg[p + 1] != P.pseg[p]) ? 1 : 0, 0};
-    };
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 agg = z;
-#pragma unroll 8
-        for (int64_t p = p1 - 1; p >= p0; --p) agg = segCombine(agg, bval(p, f));
+        bwdTiles(p0, p1, f, [&](int64_t, const Seg2& v) { agg = segCombine(agg, v); });
         s.scan[sg * BP + f] = agg;
     }
     __syncthreads();
@@ -873,12 +905,11 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     for (int i = 0; i < GE::FPL; ++i) {
         const int f = sl + i * GE::SUB;
         Seg2 run = segCombine(Seg2{s.carry[f], s.carry2[f], 0, 0}, s.scan[sg * BP + f]);
-#pragma unroll 8
-        for (int64_t p = p1 - 1; p >= p0; --p) {
-            run = segCombine(run, bval(p, f));
+        bwdTiles(p0, p1, f, [&](int64_t p, const Seg2& v) {
+            run = segCombine(run, v);
             P.R1[p * BP + f] = run.a;
             P.R2[p * BP + f] = run.b;
-        }
+        });
     }
     __syncthreads();
     return true;
