@@ -44,7 +44,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=1)
-    ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv"], default="poisson",
+    ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic"], default="poisson",
                     help="poisson: the batched-fit headline line (default); serving: the request-mix server; "
                          "lr / cox / clr-cv: the other benchmark workloads")
     ap.add_argument("--requests", type=int, default=40, help="serving: requests in the mix")
@@ -60,6 +60,8 @@ def main():
         sys.exit(2)
     if args.workload == "serving":
         return run_serving(args)
+    if args.workload == "traffic":
+        return run_traffic(args)
     if args.workload != "poisson":
         return run_other(args)
 
@@ -312,6 +314,44 @@ def run_serving(args):
         "plain_queue": plain,
     }
     print(json.dumps(res), flush=True)
+
+
+def run_traffic(args):
+    """Short run for HBM-traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
+    a 1 GiB device copy (known traffic, for scaling the counters), then one launch of
+    2 cycles of the headline batch (64 Poisson fits, teams of 16 fits x 64 workgroups)
+    whose algorithmic bytes are printed for comparison."""
+    import torch
+    from cyclops_amd import _abi
+    from cyclops_amd import data as D
+    from cyclops_amd import engine as E
+    from cyclops_amd.control import create_control
+
+    n = 1 << 28   # 2^28 float32 = 1 GiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy kernel: {n * 4} bytes read and {n * 4} bytes written")
+    del src, dst
+    torch.cuda.empty_cache()
+    os.environ["CCD_BATCH_LAYOUT"] = "16,64"
+    os.environ["CCD_CYCLES_PER_LAUNCH"] = "2"
+    d, _ = D.make_config(1, scale=args.scale)
+    rf = E.assign_folds(d.K, None, 4, _abi.SELECT_BY_ROW, 1001)
+    grid = np.exp(np.linspace(math.log(0.1), math.log(10.0), 16))
+    var_list = [float(v) for v in grid for _ in range(4)]
+    held = [f for _ in range(16) for f in range(4)]
+    eng = E.Engine(d, _abi.FP32)
+    eng.reset_kernel_stats()
+    eng.fit_batch(var_list, "normal", create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=2),
+                  held_out_fold=held, row_fold=rf)
+    ks = eng.kernel_stats()
+    eng.close()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "kfindmode_launches": ks["launches"],
+                      "kfindmode_algorithmic_bytes": ks["algorithmic_bytes"], "kfindmode_device_ms": ks["device_ms"]}),
+          flush=True)
 
 
 def run_other(args):
