@@ -1133,9 +1133,10 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     Meta cur = loadMeta(P, firstJ < P.J ? firstJ : 0, w);
     double preB = 0.0, preH = 0.0;   // thread f: beta / bound of fit f at cur.j
     bool havePre = false;
-    // chunks of at most U entries stay in registers from the column pass to the
-    // update (narrow teams only: wide ones would hold too many registers)
-    constexpr bool FAST = BP <= 4;
+    // chunks of at most UF entries stay in registers from the column pass to the
+    // update (teams of up to 4 fits, or 16 independent-row fits in the wide lane
+    // geometry; wider teams would hold too many registers)
+    constexpr bool FAST = BP <= 4 || (!GROUPED && BP <= 16);
     constexpr int UF = 4;   // entries per sub-group kept in registers
     auto fitsBatch = [&](const Meta& m) {   // every sub-group chunk of m has <= UF entries
         return (m.b - m.a + GE::NSG - 1) / GE::NSG <= UF;
