@@ -191,15 +191,15 @@ void EngineT<Real>::releaseResident() {
 static int chooseWorkgroups(int64_t K, int B, int device, bool grouped = false) {
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int cap = std::min(res::MAXG, cus);
     if (const char* e = std::getenv("CCD_RESIDENT_WG")) {
         const int v = std::atoi(e);
-        if (v > 0) return std::min(v, std::min(cus, res::MAXG));
+        if (v > 0) return std::min(v, cap);
     }
     const int64_t rowsPerWG = (B <= 2 && !grouped) ? 16384 : 4096;
     int64_t g = (K + rowsPerWG - 1) / rowsPerWG;
     if (g < 1) g = 1;
-    if (g > cus) g = cus;
-    if (g > res::MAXG) g = res::MAXG;
+    if (g > cap) g = cap;
     return (int)g;
 }
 

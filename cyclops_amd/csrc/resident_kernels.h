@@ -768,6 +768,136 @@ __device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, 
 }
 
 // Breslow risk sets of this workgroup's pids, both scans, with the cross-workgroup carries.
+// One fit per team: the same rescan with the workgroup's pids split into 4
+// contiguous wave ranges, each walked 64 consecutive pids per step (coalesced
+// loads, a wave segmented scan per step), then carried across waves and, through
+// the slots, across workgroups.
+template <typename Real, int BP>
+__device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
+                                               Shared<BP>& s) {
+    const int w = P.w;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t q0 = P.pidStart[w], q1 = P.pidStart[w + 1];
+    const int64_t cw = (q1 - q0 + WAVES - 1) / WAVES;
+    int64_t a = q0 + wave * cw, b = a + cw;
+    if (a > q1) a = q1;
+    if (b > q1) b = q1;
+    const int64_t N = P.N;
+    const Seg2 z{0.0, 0.0, 0, 0};
+    auto segc = [](const Seg2& x, const Seg2& y) { return segCombine(x, y); };
+    auto bcast63 = [](const Seg2& v) {
+        Seg2 r;
+        r.a = __shfl(v.a, 63, 64);
+        r.b = __shfl(v.b, 63, 64);
+        r.f = __shfl(v.f, 63, 64);
+        r.pad = 0;
+        return r;
+    };
+    constexpr int SU = 8;   // steps whose loads are issued together
+    // forward term of pid p (identity outside [a, b))
+    auto fwdVal = [&](int64_t p, double d, int sp, int sq) {
+        return Seg2{d, 0.0, (p > 0 && sp != sq) ? 1 : 0, 0};
+    };
+    // walks [a, b) forward; visit(p, inclusive-in-step value, valid) per lane, step
+    // totals combined into `run` by the caller's rule
+    auto fwdWalk = [&](Seg2 run, bool store) {
+        for (int64_t base = a; base < b; base += 64 * SU) {
+            double dv[SU];
+            int sp[SU], sq[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int64_t p = base + u * 64 + lane;
+                const bool ok = p < b;
+                dv[u] = ok ? P.denom[p] : 0.0;
+                sp[u] = ok ? P.pseg[p] : 0;
+                sq[u] = (ok && p > 0) ? P.pseg[p - 1] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int64_t p = base + u * 64 + lane;
+                const bool ok = p < b;
+                const Seg2 inc = waveSegScanIncl(ok ? fwdVal(p, dv[u], sp[u], sq[u]) : z);
+                if (store && ok) P.Dacc[p] = segCombine(run, inc).a;
+                run = segCombine(run, bcast63(inc));
+            }
+        }
+        return run;
+    };
+    // walks [a, b) backward (descending pids)
+    auto bwdWalk = [&](Seg2 run, bool store) {
+        for (int64_t top = b - 1; top >= a; top -= 64 * SU) {
+            double nv[SU], dv[SU];
+            int sp[SU], sn[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int64_t p = top - u * 64 - lane;
+                const bool ok = p >= a;
+                nv[u] = ok ? P.nW[p] : 0.0;
+                dv[u] = ok ? P.Dacc[p] : 1.0;
+                sp[u] = ok ? P.pseg[p] : 0;
+                sn[u] = ok ? P.pseg[p + 1] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int64_t p = top - u * 64 - lane;
+                const bool ok = p >= a;
+                Seg2 v = z;
+                if (ok) {
+                    const double r = nv[u] == 0.0 ? 0.0 : nv[u] / dv[u];
+                    v = Seg2{r, nv[u] == 0.0 ? 0.0 : r / dv[u], (p == N - 1 || sn[u] != sp[u]) ? 1 : 0, 0};
+                }
+                const Seg2 inc = waveSegScanIncl(v);
+                if (store && ok) {
+                    const Seg2 o = segCombine(run, inc);
+                    P.R1[p] = o.a;
+                    P.R2[p] = o.b;
+                }
+                run = segCombine(run, bcast63(inc));
+            }
+        }
+        return run;
+    };
+    // exchange of the workgroup totals; carry = combination of the workgroups
+    // before this one (after, for the backward pass)
+    auto exchange = [&](bool backward, Seg2& carry) -> bool {
+        if (threadIdx.x == 0) {
+            Seg2 tot = z;
+            for (int i = 0; i < WAVES; ++i) tot = segCombine(tot, s.scan[backward ? WAVES - 1 - i : i]);
+            double* o = P.slots + ((size_t)((seq + 1) & 1) * P.G + w) * SLOTW;
+            ast(o, tot.a); ast(o + 1, tot.b); ast(o + 2, (double)tot.f);
+        }
+        publish(P.epoch, ++seq, P.w);
+        if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+        const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * SLOTW;
+        const int n = backward ? P.G - 1 - w : w;
+        carry = treeReduce<Seg2>(n, [&](int q) {
+            const double* o = sl2 + (size_t)(backward ? P.G - 1 - q : q) * SLOTW;
+            return Seg2{ald(o), ald(o + 1), (int)ald(o + 2), 0};
+        }, segc, s.u.seg, z);
+        return true;
+    };
+    // ---- forward: D = segmented prefix of denom
+    Seg2 mine = fwdWalk(z, false);
+    if (lane == 0) s.scan[wave] = mine;
+    __syncthreads();
+    Seg2 carry;
+    if (!exchange(false, carry)) return false;
+    Seg2 run = Seg2{carry.a, 0.0, 0, 0};
+    for (int i = 0; i < wave; ++i) run = segCombine(run, s.scan[i]);
+    fwdWalk(run, true);
+    __syncthreads();
+    // ---- backward: R1, R2 = reverse segmented sums of nE/D, nE/D^2
+    mine = bwdWalk(z, false);
+    if (lane == 0) s.scan[wave] = mine;
+    __syncthreads();
+    if (!exchange(true, carry)) return false;
+    run = Seg2{carry.a, carry.b, 0, 0};
+    for (int i = WAVES - 1; i > wave; --i) run = segCombine(run, s.scan[i]);
+    bwdWalk(run, true);
+    __syncthreads();
+    return true;
+}
+
 template <typename Real, int BP>
 __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
                                           Shared<BP>& s) {
@@ -782,9 +912,38 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     const int64_t N = P.N;
     auto segc = [](const Seg2& a, const Seg2& b) { return segCombine(a, b); };
     const Seg2 z{0.0, 0.0, 0, 0};
+    // Exclusive scan of the sub-group aggregates s.scan[q * BP + f] in sub-group
+    // order (backward: reverse order), in place; the workgroup's total goes to its
+    // slot. One fit per workgroup (BP == 1) scans all 256 aggregates in parallel;
+    // otherwise thread f walks the sub-groups of fit f.
+    auto scanSubgroups = [&](double* slot, bool backward) {
+        if constexpr (GE::NSG == BLOCK) {
+            const int q = backward ? BLOCK - 1 - threadIdx.x : threadIdx.x;
+            Seg2 total;
+            const Seg2 ex = blockSegScanExcl(s.scan[q], total);
+            s.scan[q] = ex;
+            if (threadIdx.x == 0) {
+                double* o = slot + (size_t)w * SLOTW;
+                ast(o, total.a); ast(o + 1, total.b); ast(o + 2, (double)total.f);
+            }
+            __syncthreads();
+        } else {
+            for (int f = threadIdx.x; f < BP; f += BLOCK) {
+                Seg2 run = z;
+                for (int i = 0; i < GE::NSG; ++i) {
+                    const int q = backward ? GE::NSG - 1 - i : i;
+                    const Seg2 v = s.scan[q * BP + f];
+                    s.scan[q * BP + f] = run;
+                    run = segCombine(run, v);
+                }
+                double* o = slot + ((size_t)w * BP + f) * SLOTW;
+                ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
+            }
+        }
+    };
     // The pids of a chunk are visited TL at a time: every load of a tile is issued
     // before the first combine, so a chunk costs one memory latency per TL pids.
-    constexpr int TL = 8;
+    constexpr int TL = 16;
     // forward term of pid p: denominator, reset at a new stratum
     auto fwdTiles = [&](int64_t a, int64_t b, int f, auto visit) {
         for (int64_t p = a; p < b; p += TL) {
@@ -834,16 +993,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     }
     __syncthreads();
     double* slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
-    for (int f = threadIdx.x; f < BP; f += BLOCK) {
-        Seg2 run = z;
-        for (int q = 0; q < GE::NSG; ++q) {
-            const Seg2 v = s.scan[q * BP + f];
-            s.scan[q * BP + f] = run;
-            run = segCombine(run, v);
-        }
-        double* o = slot + ((size_t)w * BP + f) * SLOTW;
-        ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
-    }
+    scanSubgroups(slot, false);
     publish(P.epoch, ++seq, P.w);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     {
@@ -877,16 +1027,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     }
     __syncthreads();
     slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
-    for (int f = threadIdx.x; f < BP; f += BLOCK) {
-        Seg2 run = z;
-        for (int q = GE::NSG - 1; q >= 0; --q) {
-            const Seg2 v = s.scan[q * BP + f];
-            s.scan[q * BP + f] = run;
-            run = segCombine(run, v);
-        }
-        double* o = slot + ((size_t)w * BP + f) * SLOTW;
-        ast(o, run.a); ast(o + 1, run.b); ast(o + 2, (double)run.f);
-    }
+    scanSubgroups(slot, true);
     publish(P.epoch, ++seq, P.w);
     if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
     {
@@ -1086,7 +1227,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             initPids<Real, MODEL, BP>(P);
             if constexpr (MODEL == M_COX) {
                 // the pass above wrote only this workgroup's pids; the scan reads them all
-                if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+                if (!(BP == 1 ? coxRescanWaves<Real, BP>(P, seq, deadline, s) : coxRescan<Real, BP>(P, seq, deadline, s))) return;
             }
         }
     }
@@ -1306,7 +1447,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     }
                     __syncthreads();
                     if constexpr (MODEL == M_COX) {
-                        if (!coxRescan<Real, BP>(P, seq, deadline, s)) return;
+                        if (!(BP == 1 ? coxRescanWaves<Real, BP>(P, seq, deadline, s) : coxRescan<Real, BP>(P, seq, deadline, s))) return;
                     }
                 }
                 if (w == 0 && threadIdx.x == 0) {
