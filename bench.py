@@ -43,7 +43,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (1.0 = full size)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--cpu-cycles", type=int, default=3, help="CCD cycles of the CPU baseline sample (about 10 s)")
     ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic"], default="poisson",
                     help="poisson: the batched-fit headline line (default); serving: the request-mix server; "
                          "lr / cox / clr-cv: the other benchmark workloads")
