@@ -264,10 +264,10 @@ static int occupancyOf(int model, int BP) {
     }
 }
 
-// Team shapes worth timing for a batch of B fits on K rows: one lockstep team of
-// every fit, and narrower teams (4, 16 or 64 fits) of 8 to 128 workgroups, each
-// shape co-resident on the device and at least 2048 rows per workgroup; plus
-// one-workgroup teams for short data.
+// Team shapes worth timing for a batch of B fits on K rows: for up to 16 grouped
+// fits, one fit per team; one lockstep team of every fit; narrower teams (4, 16
+// or 64 fits) of 8 to 128 workgroups; each shape co-resident on the device with at
+// least 2048 rows per workgroup; plus one-workgroup teams for short data.
 template <typename Real>
 static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int device) {
     int cus = 0;
@@ -283,7 +283,13 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
         for (const Layout& q : out) if (q.Bt == L.Bt && q.T == L.T) return;
         out.push_back(L);
     };
-    if (B <= 256) add(B, chooseWorkgroups(K, B, device, model == M_CLR || model == M_COX));
+    const bool grouped = model == M_CLR || model == M_COX;
+    // a few grouped fits: one fit per team first (each team walks its own risk sets
+    // with the one-fit rescan); these probes are slow, so the first one usually decides
+    if (grouped && B <= 16)
+        for (int t : {64, 128})
+            if (K / t >= 2048) add(1, t);
+    if (B <= 256) add(B, chooseWorkgroups(K, B, device, grouped));
     for (int bt : {4, 16, 64}) {
         if (bt >= B) continue;
         for (int t : {8, 16, 24, 32, 48, 64, 96, 128})
