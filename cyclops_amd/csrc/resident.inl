@@ -266,8 +266,8 @@ static int occupancyOf(int model, int BP) {
 
 // Team shapes worth timing for a batch of B fits on K rows: for up to 16 grouped
 // fits, one fit per team; one lockstep team of every fit; narrower teams (4, 16
-// or 64 fits) of 8 to 128 workgroups; each shape co-resident on the device with at
-// least 2048 rows per workgroup; plus one-workgroup teams for short data.
+// or 64 fits) of 8 to 128 workgroups; each shape co-resident on the device with
+// 2,048 to 65,536 rows per workgroup; plus one-workgroup teams for short data.
 template <typename Real>
 static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int device) {
     int cus = 0;
@@ -293,7 +293,7 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
     for (int bt : {4, 16, 64}) {
         if (bt >= B) continue;
         for (int t : {8, 16, 24, 32, 48, 64, 96, 128})
-            if (K / t >= 2048) add(bt, t);
+            if (K / t >= 2048 && K / t <= 65536) add(bt, t);   // 2k..64k rows per workgroup
     }
     if (K <= 200000) { add(1, 1); add(4, 1); }
     return out;
