@@ -418,8 +418,8 @@ def run_other(args):
     cpu = None
     if not args.skip_cpu:
         import oracle
-        if idx == 2:   # a full Cox cycle on one core takes minutes: time its first coordinates
-            ncoord = 1500
+        if idx in (2, 3):   # a full cycle on one core takes minutes: time its first coordinates
+            ncoord = 1500 if idx == 2 else 4000
             sec = oracle.time_coordinates(d, prior_ref, ctl, ncoord, weights=w_ref)
             cpc = sec / ncoord * (d.J - 1)
             sample = (f"the first {ncoord} coordinates of the first cycle of one member fit in FP64 on one host "
