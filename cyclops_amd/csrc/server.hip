@@ -116,9 +116,13 @@ struct ccd_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    // First queued request that fits the remaining budget (caller holds m).
+    // First queued request that fits the remaining budget (caller holds m). Later
+    // requests may overtake one that does not fit yet, until the oldest has waited
+    // 2 s: then nothing overtakes it (a large request cannot starve).
     std::shared_ptr<Request> pick() {
+        const bool headWaited = !queue.empty() && now() - queue.front()->stats.submit_time > 2.0;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
+            if (headWaited && it != queue.begin()) break;
             const auto& r = *it;
             if (usedCap + r->capFraction > 1.0 + 1e-9) continue;
             size_t off = 0;
