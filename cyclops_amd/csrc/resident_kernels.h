@@ -555,31 +555,6 @@ __device__ __forceinline__ void combinePlain(const Mon (&acc)[GE::FPL], Mon* sMo
     __syncthreads();
 }
 
-// Ordered combine of the G workgroup partials of fit f (slots) -> thread 0 gets it.
-__device__ __forceinline__ Mon reduceSlots(const double* slot, int G, int BP, int f, Mon* sRed) {
-    int P2 = 1;
-    while (P2 < G) P2 <<= 1;
-    for (int w = threadIdx.x; w < P2; w += BLOCK) {
-        if (w < G) {
-            const double* s = slot + ((size_t)w * BP + f) * SLOTW;
-            sRed[w] = Mon{s[0], s[1], s[2], s[3], s[4], s[5]};
-        } else {
-            sRed[w] = monZero();
-        }
-    }
-    __syncthreads();
-    for (int s = 1; s < P2; s <<= 1) {
-        for (int q = threadIdx.x; q < P2 / (2 * s); q += BLOCK) {
-            const int a = 2 * s * q;
-            sRed[a] = monCombine(sRed[a], sRed[a + s]);
-        }
-        __syncthreads();
-    }
-    const Mon r = sRed[0];
-    __syncthreads();
-    return r;
-}
-
 __device__ __forceinline__ double priorDeltaDev(int type, double variance, double g, double h, double beta) {
     if (type == 0) return -(g / h);
     if (type == 2) return -(g + beta / variance) / (h + 1.0 / variance);
@@ -616,21 +591,44 @@ __device__ __forceinline__ double logDensityDev(int type, double variance, doubl
 namespace ccd {
 namespace res {
 
-// Ordered tree combine of n items (loaded by ld(i)) in LDS; every thread gets the result.
+__device__ __forceinline__ Mon shflDown(const Mon& v, int d) {
+    return Mon{__shfl_down(v.g, d, 64), __shfl_down(v.h, d, 64), __shfl_down(v.l, d, 64),
+               __shfl_down(v.m, d, 64), __shfl_down(v.t, d, 64), __shfl_down(v.r, d, 64)};
+}
+__device__ __forceinline__ Seg2 shflDown(const Seg2& v, int d) {
+    Seg2 r;
+    r.a = __shfl_down(v.a, d, 64);
+    r.b = __shfl_down(v.b, d, 64);
+    r.f = __shfl_down(v.f, d, 64);
+    r.pad = 0;
+    return r;
+}
+
+// Ordered combine of n <= 2 * BLOCK items (loaded by ld(i)); every thread gets the
+// result. Thread t takes items 2t and 2t+1, each wave combines its threads in a
+// pairwise tree of shuffles (lane l with lane l + d: always the next block, so the
+// order of the items is kept), and the wave results are combined in order.
 template <class T, class Load, class Comb>
 __device__ __forceinline__ T treeReduce(int n, Load ld, Comb cb, T* sBuf, T ident) {
-    int P2 = 1;
-    while (P2 < n) P2 <<= 1;
-    for (int i = threadIdx.x; i < P2; i += BLOCK) sBuf[i] = (i < n) ? ld(i) : ident;
-    __syncthreads();
-    for (int s = 1; s < P2; s <<= 1) {
-        for (int q = threadIdx.x; q < P2 / (2 * s); q += BLOCK) {
-            const int a = 2 * s * q;
-            sBuf[a] = cb(sBuf[a], sBuf[a + s]);
-        }
-        __syncthreads();
+    const int i0 = 2 * threadIdx.x;
+    T v = ident;
+    if (i0 < n) v = ld(i0);
+    if (i0 + 1 < n) v = cb(v, ld(i0 + 1));
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T o = shflDown(v, d);
+        if ((lane & (2 * d - 1)) == 0) v = cb(v, o);
     }
-    const T r = sBuf[0];
+    if (lane == 0) sBuf[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T r = sBuf[0];
+        for (int q = 1; q < BLOCK / 64; ++q) r = cb(r, sBuf[q]);
+        sBuf[WAVES] = r;
+    }
+    __syncthreads();
+    const T r = sBuf[WAVES];
     __syncthreads();
     return r;
 }
