@@ -25,6 +25,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+# HBM bytes moved by kFindMode per algorithmic byte on the default workload (64 fits,
+# scale 1): two rocprofv3 --pmc passes over this script's own timed step, FETCH_SIZE x 2
+# plus WRITE_SIZE x 1 (the factors a 1 GiB device copy reads back), summed over the
+# step's launches and divided by the bytes the kernel counts.  The measurement and its
+# commands are in profiles/kFindMode_poisson_K1e6_J2e4_batch64_traffic.md.
+TRAFFIC_PER_ALGORITHMIC_BYTE = 1.2335
 
 
 def log(msg):
@@ -118,6 +124,8 @@ def main():
     launch_ms = ks["device_ms"] / max(ks["launches"], 1)
     bytes_per_launch = ks["algorithmic_bytes"] / max(ks["launches"], 1)
     achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9 if launch_ms > 0 else 0.0
+    measured = args.batch == 64 and args.folds == 4 and args.scale == 1.0
+    traffic = TRAFFIC_PER_ALGORITHMIC_BYTE * bytes_per_launch if measured else None
 
     # one member of the mix: variance closest to 1.0, fold 0
     ref_i = int(np.argmin([abs(math.log(v)) + 10 * h for v, h in zip(var_list, held)]))
@@ -201,7 +209,7 @@ def main():
         "plain_per_fit": plain,
         "speedup_vs_plain": (fits_per_sec / plain["fits_per_sec"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "ccd::res::kFindMode<float, POISSON, BP>", "launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
