@@ -96,6 +96,7 @@ class BatchArgs(C.Structure):
         ("variances", C.POINTER(C.c_double)),
         ("held_out_fold", C.POINTER(C.c_int32)),
         ("row_fold", C.POINTER(C.c_int32)),
+        ("fixed_index", C.POINTER(C.c_int32)),
     ]
 
 

@@ -10,7 +10,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcyclops_amd.so")
-SOURCES = ["engine.hip", "api.hip", "server.hip", "resident_logistic.hip", "resident_poisson.hip",
+SOURCES = ["engine.hip", "api.hip", "server.hip", "profile.hip", "resident_logistic.hip", "resident_poisson.hip",
            "resident_clr.hip", "resident_cox.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

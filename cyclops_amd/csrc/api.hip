@@ -462,6 +462,28 @@ int ccd_fit_batch(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args
     });
 }
 
+int ccd_fisher_information(ccd_handle h, const double* beta, const int32_t* indices, int32_t m, double* out) {
+    return guard([&] {
+        if (!beta || (m > 0 && (!indices || !out))) throw std::invalid_argument("null argument");
+        if (m < 0) throw std::invalid_argument("m must be >= 0");
+        E(h).fisherInformation(beta, indices, m, out);
+    });
+}
+
+int ccd_profile(ccd_handle h, const ccd_fit_args* args, const double* beta, const int32_t* indices, int32_t m,
+                double threshold, int include_penalty, int override_no_regularization, double* lower,
+                double* upper, int32_t* evaluations) {
+    return guard([&] {
+        if (!args || !beta || (m > 0 && (!indices || !lower || !upper || !evaluations)))
+            throw std::invalid_argument("null argument");
+        if (m < 0) throw std::invalid_argument("m must be >= 0");
+        if (!(threshold > 0.0)) throw std::invalid_argument("threshold must be > 0");
+        ccd::checkArgs(E(h), *args);
+        ccd::profileModel(E(h), *args, beta, indices, m, threshold, include_penalty != 0,
+                          override_no_regularization != 0, lower, upper, evaluations);
+    });
+}
+
 int ccd_cross_validate(ccd_handle h, const ccd_fit_args* common, const ccd_cv_args* cv, double* grid_values,
                        double* pred_loglik, double* beta, ccd_fit_result* final_result, ccd_cv_result* cv_result) {
     return guard([&] {

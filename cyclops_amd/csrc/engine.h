@@ -84,8 +84,14 @@ public:
 
     // Resident / batched fits (resident.hip).
     virtual void fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) = 0;
+    // autotune: time candidate team shapes on the first batch of each size (callers
+    // issuing many small batches of varying size, such as profiling, pass false)
     virtual void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas,
-                          ccd_fit_result* r) = 0;
+                          ccd_fit_result* r, bool autotune = true) = 0;
+
+    // Fisher information of columns idx[0..m) at beta, unit row weights (the engine's
+    // weights are reset): out[m][m]. fisher.inl.
+    virtual void fisherInformation(const double* beta, const int32_t* idx, int m, double* out) = 0;
 
     KernelCounters counters;
     bool timing = false;
@@ -106,6 +112,12 @@ ResidentNeeds residentNeeds(const ccd_data_desc& d, int device);
 // Host CCD driver over an EngineBase: CyclicCoordinateDescent::update/findMode
 // (CyclicCoordinateDescent.cpp:586-615, 909-1216). This is the plain per-fit path.
 void fitPlain(EngineBase& eng, const ccd_fit_args& a, double* beta, ccd_fit_result& r);
+
+// Profile-likelihood limits (CcdInterface::profileModel, CcdInterface.cpp:262-434): all
+// bracket and root searches advance together, one batched fit per round (profile.hip).
+void profileModel(EngineBase& eng, const ccd_fit_args& a, const double* beta, const int32_t* idx, int m,
+                  double threshold, bool includePenalty, bool overrideNoRegularization, double* lower,
+                  double* upper, int32_t* evaluations);
 
 // Fold assignment of CrossValidationSelector (drivers/CrossValidationSelector.cpp:19-189).
 void assignFolds(int64_t K, const int32_t* pid, int fold, int selector, int64_t seed, int32_t* rowFold);

@@ -649,7 +649,10 @@ public:
 
     // resident.hip
     void fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) override;
-    void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas, ccd_fit_result* r) override;
+    void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas, ccd_fit_result* r,
+                  bool autotune) override;
+    // fisher.inl
+    void fisherInformation(const double* beta, const int32_t* idx, int m, double* out) override;
     void releaseResident();
 
     std::shared_ptr<DevData<Real>> D;
@@ -679,6 +682,7 @@ public:
 }  // namespace ccd
 
 #include "resident.inl"
+#include "fisher.inl"
 
 namespace ccd {
 

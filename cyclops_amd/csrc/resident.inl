@@ -143,7 +143,7 @@ struct ResWork {
     double *beta = nullptr, *hDelta = nullptr, *var = nullptr, *initBound = nullptr;
     unsigned char *exclude = nullptr, *fixed = nullptr;
     double *xjyTot = nullptr, *xjyFold = nullptr, *xjyW = nullptr, *fixedTerm = nullptr;
-    int *held = nullptr, *rowFold = nullptr;
+    int *held = nullptr, *rowFold = nullptr, *fixJ = nullptr;
     int *flag = nullptr, *iters = nullptr, *done = nullptr, *ctlIter = nullptr, *ctlFlag = nullptr;
     double* ctlLast = nullptr;
     double* ll = nullptr;
@@ -160,7 +160,7 @@ struct ResWork {
     void freeBatch() {
         dfree(xb); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
-        dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold);
+        dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
         dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw); dfree(teams); dfree(info);
         B = BP = nTeams = 0;
@@ -305,7 +305,7 @@ template <typename Real>
 static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
                         const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
                         int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps,
-                        int& nLaunches, Layout L) {
+                        int& nLaunches, Layout L, const int32_t* fixJ = nullptr) {
     auto& D = *E.D;
     hipStream_t s = E.s;
     const int64_t K = D.K;
@@ -419,6 +419,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         W.xjyW = dalloc<double>(J, Bb);
         W.fixedTerm = dalloc<double>(B, Bb);
         W.held = dalloc<int>(B, Bb);
+        W.fixJ = dalloc<int>(B, Bb);
         W.rowFold = dalloc<int>(K, Bb);
         W.flag = dalloc<int>(B, Bb);
         W.iters = dalloc<int>(B, Bb);
@@ -471,6 +472,9 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     std::vector<int32_t> hh(B, -1);
     if (held) std::copy(held, held + B, hh.begin());
     h2d(W.held, hh.data(), B, s);
+    std::vector<int32_t> fj(B, -1);
+    if (fixJ) std::copy(fixJ, fixJ + B, fj.begin());
+    h2d(W.fixJ, fj.data(), B, s);
     if (rowFold) h2d(W.rowFold, rowFold, K, s);
     if (kwHost) h2dConvert(W.kw, kwHost, K, s);
     {   // XjY: total (and per fold), or weighted for a single weighted fit
@@ -553,6 +557,7 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         }
         Q.beta = W.beta + f0 * J; Q.hDelta = W.hDelta + f0 * J;
         Q.var = W.var + f0; Q.initBound = W.initBound + f0; Q.held = W.held + f0;
+        Q.fixJ = fixJ ? W.fixJ + f0 : nullptr;
         Q.fixedTermFit = W.fixedTerm + f0;
         Q.flag = W.flag + f0; Q.done = W.done + f0; Q.iters = W.iters + f0; Q.ll = W.ll + f0;
         Q.ctlIter = W.ctlIter + f0; Q.ctlFlag = W.ctlFlag + f0; Q.ctlLast = W.ctlLast + f0;
@@ -682,7 +687,8 @@ void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_res
 }
 
 template <typename Real>
-void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas, ccd_fit_result* r) {
+void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas, ccd_fit_result* r,
+                             bool autotune) {
     CCD_HIP(hipSetDevice(D->device));
     const auto t0 = std::chrono::steady_clock::now();
     const int J = D->J;
@@ -698,6 +704,10 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     if (bargs.held_out_fold && !bargs.row_fold)
         for (int f = 0; f < n; ++f)
             if (bargs.held_out_fold[f] >= 0) throw std::invalid_argument("held_out_fold needs row_fold");
+    if (bargs.fixed_index)
+        for (int f = 0; f < n; ++f)
+            if (bargs.fixed_index[f] < -1 || bargs.fixed_index[f] >= J)
+                throw std::invalid_argument("fixed_index must be -1 or a column index");
     for (int f = 0; f < n; ++f) {
         if (a.prior_type != CCD_PRIOR_NONE && !(bargs.variances[f] > 0))
             throw std::invalid_argument("variances must be > 0");
@@ -713,7 +723,8 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     // on one timed cycle of these fits (kept per batch size), else the default.
     bool tunedOk = false;
     Layout LT;
-    if (n > 2 && !std::getenv("CCD_BATCH_LAYOUT") && !(std::getenv("CCD_AUTOTUNE") && std::atoi(std::getenv("CCD_AUTOTUNE")) == 0)) {
+    if (n > 2 && autotune && !std::getenv("CCD_BATCH_LAYOUT") &&
+        !(std::getenv("CCD_AUTOTUNE") && std::atoi(std::getenv("CCD_AUTOTUNE")) == 0)) {
         if (!res) res = new ResWork<Real>();
         auto& W = *static_cast<ResWork<Real>*>(res);
         auto it = W.tuned.find(n);
@@ -740,7 +751,8 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
                     double ms = 0, bytes = 0, steps = 0;
                     int nl = 0;
                     runResident<Real>(*this, one, n, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(),
-                                      bb.data(), fl.data(), it2.data(), l.data(), ms, bytes, steps, nl, L);
+                                      bb.data(), fl.data(), it2.data(), l.data(), ms, bytes, steps, nl, L,
+                                      bargs.fixed_index);
                     if (std::getenv("CCD_PROGRESS"))
                         std::fprintf(stderr, "[ccd] autotune %d fits: %d fits x %d workgroups per team: %.1f ms per cycle\n",
                                      n, L.Bt, L.T, ms);
@@ -761,19 +773,21 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
             const int B = (int)std::min(chunk, pending.size() - c0);
             const Layout L = whole ? LT : chooseLayout(D->K, B, D->device, grouped);
             std::vector<double> v(B), bd(B), bb((size_t)B * J), l(B);
-            std::vector<int32_t> h(B);
+            std::vector<int32_t> h(B), fj(B);
             std::vector<int> fl(B), it(B);
             for (int i = 0; i < B; ++i) {
                 const int f = pending[c0 + i];
                 v[i] = bargs.variances[f];
                 h[i] = bargs.held_out_fold ? bargs.held_out_fold[f] : -1;
+                fj[i] = bargs.fixed_index ? bargs.fixed_index[f] : -1;
                 bd[i] = bound[f];
                 std::copy(betas + (size_t)f * J, betas + (size_t)(f + 1) * J, bb.begin() + (size_t)i * J);
             }
             double ms = 0, bytes = 0, steps = 0;
             int nl = 0;
             runResident<Real>(*this, a, B, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(), bb.data(),
-                              fl.data(), it.data(), l.data(), ms, bytes, steps, nl, L);
+                              fl.data(), it.data(), l.data(), ms, bytes, steps, nl, L,
+                              bargs.fixed_index ? fj.data() : nullptr);
             if (timing) {
                 counters.deviceMs += ms;
                 counters.launches += nl;

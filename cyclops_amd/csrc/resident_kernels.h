@@ -66,6 +66,7 @@ struct Params {
     const Real* kw;       // [K] (single fit) or null
     const int* rowFold;   // [K] or null
     const int* held;      // [B]
+    const int* fixJ;      // [B] column held at its starting value in fit f (-1: none), or null
     // fit state
     Real* xb;             // [K*BP]
     double* denom;        // [N*BP] (CLR/Cox)
@@ -1307,6 +1308,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     h = GROUPED ? m.h - m.l : m.h;
                 }
                 if (cDone[f]) { o[0] = 0.0; return; }
+                if (P.fixJ && P.fixJ[f] == j) { o[0] = 0.0; return; }   // setFixedBeta for this fit only
                 const double b = s.beta[f];
                 double hd = s.hd[f];
                 if (h < 0.0) { g = 0.0; h = 0.0; }

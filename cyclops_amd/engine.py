@@ -67,6 +67,9 @@ def lib():
         L.ccd_cross_validate.argtypes = [vp, C.POINTER(_abi.FitArgs), C.POINTER(_abi.CvArgs), dp, dp, dp,
                                          C.POINTER(_abi.FitResult), C.POINTER(_abi.CvResult)]
         L.ccd_cv_assign_folds.argtypes = [C.c_int64, C.POINTER(i32), i32, i32, C.c_int64, C.POINTER(i32)]
+        L.ccd_fisher_information.argtypes = [vp, dp, C.POINTER(i32), i32, dp]
+        L.ccd_profile.argtypes = [vp, C.POINTER(_abi.FitArgs), dp, C.POINTER(i32), i32, C.c_double, C.c_int,
+                                  C.c_int, dp, dp, C.POINTER(i32)]
         L.ccd_get_kernel_stats.argtypes = [vp, C.POINTER(_abi.KernelStats)]
         L.ccd_reset_kernel_stats.argtypes = [vp]
         L.ccd_estimate_footprint.argtypes = [C.POINTER(_abi.DataDesc), dp]
@@ -211,7 +214,7 @@ class Engine:
         return beta, res.as_dict()
 
     def fit_batch(self, variances, prior_type="laplace", control=None, held_out_fold=None, row_fold=None,
-                  betas0=None, exclude=()):
+                  betas0=None, exclude=(), fixed_index=None):
         control = control or create_control()
         n = len(variances)
         prior = create_prior(prior_type, float(variances[0]), exclude=exclude)
@@ -224,6 +227,8 @@ class Engine:
         b.variances = _d(var)
         b.held_out_fold = _abi.ptr(hof, C.c_int32)
         b.row_fold = _abi.ptr(rf, C.c_int32)
+        fi = None if fixed_index is None else np.ascontiguousarray(fixed_index, dtype=np.int32)
+        b.fixed_index = _abi.ptr(fi, C.c_int32)
         betas = np.zeros((n, self.data.J)) if betas0 is None else np.array(betas0, dtype=np.float64)
         betas = np.ascontiguousarray(betas)
         results = (_abi.FitResult * n)()
@@ -269,6 +274,32 @@ class Engine:
                 "result": res.as_dict(), "best_variance": cvr.best_variance, "best_value": cvr.best_value,
                 "seconds": cvr.seconds, "n_fits": cvr.n_fits}
 
+    # --- variance and profile likelihood ------------------------------------------
+    def fisher_information(self, beta, indices):
+        """Fisher information of columns `indices` at beta ([m][m]; unit row weights):
+        cyclopsGetFisherInformation (RcppCyclopsInterface.cpp:176-200)."""
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        m = len(idx)
+        out = np.zeros((m, m))
+        check(lib().ccd_fisher_information(self.h, _d(b), idx.ctypes.data_as(C.POINTER(C.c_int32)), m, _d(out)))
+        return out
+
+    def profile(self, prior, control, beta, indices, threshold, include_penalty=True,
+                override_no_regularization=False):
+        """Profile-likelihood limits of columns `indices` around the mode beta:
+        cyclopsProfileModel (CcdInterface::profileModel). Returns (lower, upper, evaluations)."""
+        hold = FitArgsHolder(self.data.J, prior, control, intercept_index=intercept_index(self.data))
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        m = len(idx)
+        lo, up = np.zeros(m), np.zeros(m)
+        ev = np.zeros(m, dtype=np.int32)
+        check(lib().ccd_profile(self.h, C.byref(hold.args), _d(b), idx.ctypes.data_as(C.POINTER(C.c_int32)), m,
+                                float(threshold), int(bool(include_penalty)), int(bool(override_no_regularization)),
+                                _d(lo), _d(up), ev.ctypes.data_as(C.POINTER(C.c_int32))))
+        return lo, up, ev
+
     def kernel_stats(self):
         st = _abi.KernelStats()
         check(lib().ccd_get_kernel_stats(self.h, C.byref(st)))
@@ -287,6 +318,52 @@ def fit_cyclops_model(data, prior=None, control=None, weights=None, fixed=None, 
         return eng.fit(prior, control, weights=weights, fixed=fixed, beta0=beta0, mode=mode)
     finally:
         eng.close()
+
+
+def _covariates(data, covariates):
+    """Column indices of the covariates R names by number or label (.checkCovariates);
+    None means every covariate (the offset column is not one)."""
+    first = 1 if data.has_offset else 0
+    if covariates is None:
+        return list(range(first, data.J))
+    out = []
+    for c in covariates:
+        if isinstance(c, str):
+            if not data.names or c not in data.names:
+                raise CyclopsError(f"Variable {c} not found.")
+            out.append(data.names.index(c))
+        else:
+            out.append(int(c))
+    return out
+
+
+def get_fisher_information(engine, beta, covariates=None):
+    """.cyclopsGetFisherInformation (R/RcppExports.R:56-58) at the fitted beta."""
+    return engine.fisher_information(beta, _covariates(engine.data, covariates))
+
+
+def vcov(engine, beta):
+    """vcov.cyclopsFit (R/ModelFit.R:884-894): the inverse Fisher information of every covariate."""
+    return np.linalg.inv(get_fisher_information(engine, beta))
+
+
+def get_ses(engine, beta, covariates=None):
+    """getSEs (R/ModelFit.R:726-736): sqrt of the diagonal of the inverse Fisher information."""
+    return np.sqrt(np.diag(np.linalg.inv(get_fisher_information(engine, beta, covariates))))
+
+
+def confint(engine, prior, control, beta, parm=None, level=0.95, override_no_regularization=False,
+            include_penalty=True):
+    """confint.cyclopsFit (R/ModelFit.R:762-799): profile-likelihood limits at `level`
+    (threshold qchisq(level, 1) / 2). Returns an array [m][3]: column index, lower, upper;
+    NaN where a limit is not bracketed (R reports NA)."""
+    from scipy.stats import chi2
+    if level < 0.01 or level > 0.99:
+        raise CyclopsError("level must be between 0 and 1")
+    cols = _covariates(engine.data, parm)
+    lo, up, _ = engine.profile(prior, control, beta, cols, chi2.ppf(level, 1) / 2.0, include_penalty,
+                               override_no_regularization)
+    return np.column_stack([np.array(cols, dtype=np.float64), lo, up])
 
 
 def assign_folds(K, pid, fold, selector, seed):

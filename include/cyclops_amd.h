@@ -201,15 +201,45 @@ int ccd_fit(ccd_handle h, const ccd_fit_args* args, int mode,
  *   w_f[k] = (row_fold[k] == held_out_fold[f]) ? 0 : 1     (held_out_fold[f] >= 0)
  *   w_f[k] = 1                                              (held_out_fold[f] <  0)
  * Everything else comes from `common` (its `weights` field must be NULL).
+ * fixed_index[f] >= 0 holds that column of fit f at its starting value, as
+ * CyclicCoordinateDescent::setFixedBeta (CyclicCoordinateDescent.cpp:384-386)
+ * does for one fit (the profile-likelihood evaluations of CcdInterface.cpp:237-245).
  * betas: [n_fits][J], in (starting values) and out. */
 typedef struct ccd_batch_args {
     int32_t        n_fits;
     const double*  variances;      /* [n_fits] */
     const int32_t* held_out_fold;  /* [n_fits] or NULL (= all -1) */
     const int32_t* row_fold;       /* [K] fold of each row, or NULL */
+    const int32_t* fixed_index;    /* [n_fits] column held fixed in fit f, -1 for none; or NULL */
 } ccd_batch_args;
 int ccd_fit_batch(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args* batch,
                   double* betas, ccd_fit_result* results);
+
+/* ---- variance and profile likelihood ---------------------------------------
+ * Fisher information of the columns indices[0..m) at beta:
+ * CyclicCoordinateDescent::computeFisherInformation (CyclicCoordinateDescent.cpp:1288-1300)
+ * over ModelSpecifics::computeFisherInformation (ModelSpecifics.hpp:1204-1352), with
+ * unit row weights (the reference refuses cross-validation weights, :1206-1207; the
+ * engine's weights are reset to none). R's vcov / getSEs invert it
+ * (R/ModelFit.R:732-733, 887-888). out: [m][m], row-major, symmetric. */
+int ccd_fisher_information(ccd_handle h, const double* beta, const int32_t* indices, int32_t m, double* out);
+
+/* Profile-likelihood confidence limits: CcdInterface::profileModel
+ * (CcdInterface.cpp:262-434, its NEW_PROFILE branch) with OptimizationProfile (:223-260) and RZeroIn
+ * (utils/RZeroIn.h:39-74, 183-288: bracketSignChange, then Brent's zeroin with tolerance 1e-3).
+ * beta is the mode (the fit of `args`); for column indices[i], lower[i] / upper[i]
+ * are where the log-likelihood maximised with that column held fixed (plus the
+ * log prior when include_penalty) falls `threshold` below its value at the mode
+ * (R's confint: threshold = qchisq(level, 1) / 2, R/ModelFit.R:772). NaN where no
+ * sign change is bracketed in 21 steps. evaluations[i]: profile fits for column i.
+ * Every bound search starts from the mode, as the reference's per-thread clones do
+ * (CcdInterface.cpp:319-325), and warm-starts each fit from its previous one; all
+ * searches advance together, each round of evaluations being one batched fit.
+ * A profiled column that carries a prior is refused unless override_no_regularization
+ * (:300-304). args->weights must be NULL. */
+int ccd_profile(ccd_handle h, const ccd_fit_args* args, const double* beta, const int32_t* indices, int32_t m,
+                double threshold, int include_penalty, int override_no_regularization,
+                double* lower, double* upper, int32_t* evaluations);
 
 /* ---- cross-validation (grid search) --------------------------------------
  * CrossValidationSelector (drivers/CrossValidationSelector.cpp:19-189) with

@@ -161,6 +161,46 @@ def fit(data, prior=None, control=None, weights=None, fixed=None, beta0=None):
     return beta, res.as_dict()
 
 
+def fisher_information(data, beta, indices):
+    """Fisher information of columns `indices` at beta, unit weights: [m][m] (FP64 CPU)."""
+    L = lib()
+    L.orc_fisher_information.restype = C.c_int
+    L.orc_fisher_information.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int,
+                                         C.POINTER(C.c_double)]
+    eng = OracleEngine(data)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    idx = np.ascontiguousarray(indices, dtype=np.int32)
+    m = len(idx)
+    out = np.zeros((m, m))
+    rc = L.orc_fisher_information(eng.h, _d(b), idx.ctypes.data_as(C.POINTER(C.c_int)), m, _d(out))
+    if rc != 0:
+        raise RuntimeError(L.orc_last_error().decode())
+    return out
+
+
+def profile(data, prior, control, beta, indices, threshold=1.920729, include_penalty=True):
+    """Profile-likelihood limits (CcdInterface::profileModel, one CCD per bound started at
+    the mode): returns (lower, upper, evaluations) per column of `indices`."""
+    L = lib()
+    L.orc_profile.restype = C.c_int
+    L.orc_profile.argtypes = [C.c_void_p, C.POINTER(_abi.FitArgs), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                              C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int)]
+    hold = _args(data, prior, control)
+    eng = OracleEngine(data)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    idx = np.ascontiguousarray(indices, dtype=np.int32)
+    m = len(idx)
+    lo, up = np.zeros(m), np.zeros(m)
+    ev = np.zeros(m, dtype=np.int32)
+    rc = L.orc_profile(eng.h, C.byref(hold.args), _d(b), idx.ctypes.data_as(C.POINTER(C.c_int)), m,
+                       float(threshold), int(bool(include_penalty)), _d(lo), _d(up),
+                       ev.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        raise RuntimeError(L.orc_last_error().decode())
+    return lo, up, ev
+
+
 def prior_delta(prior_type, variance, g, h, beta):
     """The prior's coordinate step (CovariatePrior.h getDelta) for gradient g, Hessian h."""
     L = lib()

@@ -45,6 +45,7 @@
 #include "../include/cyclops_amd.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -477,6 +478,42 @@ public:
 
     const Data& d;
     int64_t K, N;
+    // ModelSpecifics::computeFisherInformationImpl (ModelSpecifics.hpp:1269-1352) for
+    // every pair CyclicCoordinateDescent::computeFisherInformation asks for
+    // (CyclicCoordinateDescent.cpp:1288-1300); remainingStats with unit weights first.
+    // Increments: Logistic (ModelSpecifics.h:521-535) w (g - g^2), g = e / denom;
+    // PoissonRegression (:1497-1507) w e; Survival (:488-497, CLR and Cox) w e / denom of
+    // the row's group, less the strata cross terms sum_n C_a[n] C_b[n] / denom[n]^2 with
+    // C_a[n] = sum of x_a e over the rows of group n (getSubjectSpecificHessianIterator,
+    // ModelSpecifics.hpp:1245-1267). Plain loops over dense copies of the columns.
+    void fisher(const int* idx, int m, double* out) {
+        std::vector<std::vector<double>> X(m, std::vector<double>(K, 0.0));
+        for (int a = 0; a < m; ++a) forEach(d, idx[a], [&](int64_t k, double x) { X[a][k] = x; });
+        std::vector<double> w(K);
+        for (int64_t k = 0; k < K; ++k) {
+            if (d.model == LOGISTIC) { const double g = e[k] / denom[k]; w[k] = g - g * g; }
+            else if (d.model == POISSON) w[k] = e[k];
+            else w[k] = e[k] / denom[pid[k]];
+        }
+        std::vector<std::vector<double>> C;
+        if (!independent()) {
+            C.assign(m, std::vector<double>(N, 0.0));
+            for (int a = 0; a < m; ++a)
+                for (int64_t k = 0; k < K; ++k) C[a][pid[k]] += X[a][k] * e[k];
+        }
+        for (int a = 0; a < m; ++a)
+            for (int b = a; b < m; ++b) {
+                double v = 0.0;
+                for (int64_t k = 0; k < K; ++k) v += w[k] * X[a][k] * X[b][k];
+                if (!independent()) {
+                    double cross = 0.0;
+                    for (int64_t n = 0; n < N; ++n) cross += C[a][n] * C[b][n] / (denom[n] * denom[n]);
+                    v -= cross;
+                }
+                out[(size_t)a * m + b] = out[(size_t)b * m + a] = v;
+            }
+    }
+
     std::vector<double> xb, e, denom, numer, numer2, accDenom, nW, kW, XjY;
     std::vector<int> pid, accReset;
     std::vector<std::vector<int>> sparseIdx;
@@ -815,6 +852,125 @@ double orc_time_coordinates(void* hv, const ccd_fit_args* a, int n) {
     ccd.initialBound = a->initial_bound;
     ccd.lazy();   // set-up outside the timed region
     return ccd.timeCoordinates(n);
+}
+
+/* Fisher information of columns idx[0..m) at beta, unit weights: out[m][m]. */
+int orc_fisher_information(void* hv, const double* beta, const int* idx, int m, double* out) {
+    try {
+        auto* h = static_cast<orc_handle*>(hv);
+        auto* ms = h->ms;
+        ms->setWeights(nullptr, false);
+        ms->zeroXBeta();
+        for (int j = 0; j < h->data.J; ++j) ms->axpyXBeta(beta[j], j);
+        ms->remainingStats(false);
+        ms->fisher(idx, m, out);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_orc_err = ex.what();
+        return 1;
+    }
+}
+
+/* CcdInterface::profileModel (CcdInterface.cpp:262-434) for one thread per bound:
+ * each bound search runs on its own CCD started at the mode (the reference's clones,
+ * :319-325), evaluating OptimizationProfile::objective (:238-249) one point at a time
+ * inside RZeroIn::bracketSignChange (utils/RZeroIn.h:50-74) and Brent's zeroin
+ * (R_zeroin2, :183-288; tolerance 1e-3 as CcdInterface.cpp:345). */
+int orc_profile(void* hv, const ccd_fit_args* a, const double* beta, const int* idx, int m, double threshold,
+                int includePenalty, double* lower, double* upper, int* evals) {
+    try {
+        auto* h = static_cast<orc_handle*>(hv);
+        const int J = h->data.J;
+        std::vector<double> mode(beta, beta + J);
+        if (h->data.hasOffset) mode[0] = 1.0;
+        double maxValue;
+        {
+            orc::CCD base(h->data, *h->ms);
+            orc::configure(base, a);
+            base.setBeta(mode.data());
+            maxValue = base.logLik() + (includePenalty ? base.logPrior() : 0.0);
+        }
+        for (int i = 0; i < m; ++i) {
+            const int col = idx[i];
+            for (int side = 0; side < 2; ++side) {
+                const double dir = side == 0 ? 1.0 : -1.0;
+                orc::CCD ccd(h->data, *h->ms);
+                orc::configure(ccd, a);
+                ccd.setBeta(mode.data());
+                int n = 0;
+                auto objective = [&](double x) {
+                    ++n;
+                    ccd.beta[col] = x;
+                    ccd.xBetaKnown = false;
+                    ccd.suffKnown = false;
+                    ccd.fixBeta[col] = 1;
+                    ccd.update(*a);
+                    ccd.fixBeta[col] = 0;
+                    return ccd.logLik() + threshold - maxValue + (includePenalty ? ccd.logPrior() : 0.0);
+                };
+                const double x0 = mode[col], obj0 = threshold;
+                // bracket: steps of 0.1, 0.2, 0.4, ... times max(|x0|, 0.01)
+                const double disp = std::max(std::abs(x0), 0.01);
+                double mult = 0.1, x1 = x0, o1 = obj0, pt;
+                int steps = 0;
+                bool bracketed = true;
+                while (o1 > 0) {
+                    x1 = x0 + dir * disp * mult;
+                    o1 = objective(x1);
+                    if (std::isnan(o1) || steps > 20) { bracketed = false; break; }
+                    ++steps;
+                    mult *= 2.0;
+                }
+                if (!bracketed) {
+                    pt = std::nan("");
+                } else {
+                    // Brent's method on [x0, x1] (ends ordered), as published
+                    double aa = x0, bb = x1, fa = obj0, fb = o1;
+                    if (bb < aa) { std::swap(aa, bb); std::swap(fa, fb); }
+                    double cc = aa, fc = fa;
+                    const double tol = 1e-3;
+                    if (fa == 0.0) pt = aa;
+                    else if (fb == 0.0) pt = bb;
+                    else {
+                        pt = std::nan("");
+                        for (int it = 0; it <= 1000; ++it) {
+                            const double prev = bb - aa;
+                            if (std::abs(fc) < std::abs(fb)) { aa = bb; bb = cc; cc = aa; fa = fb; fb = fc; fc = fa; }
+                            const double tolAct = 2.0 * DBL_EPSILON * std::abs(bb) + tol / 2.0;
+                            double step = (cc - bb) / 2.0;
+                            if (std::abs(step) <= tolAct || fb == 0.0) { pt = bb; break; }
+                            if (std::abs(prev) >= tolAct && std::abs(fa) > std::abs(fb)) {
+                                const double cb = cc - bb;
+                                double p, q;
+                                if (aa == cc) { const double t = fb / fa; p = cb * t; q = 1.0 - t; }
+                                else {
+                                    const double qq = fa / fc, r = fb / fc, t = fb / fa;
+                                    p = t * (cb * qq * (qq - r) - (bb - aa) * (r - 1.0));
+                                    q = (qq - 1.0) * (r - 1.0) * (t - 1.0);
+                                }
+                                if (p > 0.0) q = -q; else p = -p;
+                                if (p < 0.75 * cb * q - std::abs(tolAct * q) / 2.0 && p < std::abs(prev * q / 2.0))
+                                    step = p / q;
+                            }
+                            if (std::abs(step) < tolAct) step = step > 0.0 ? tolAct : -tolAct;
+                            aa = bb; fa = fb;
+                            bb += step;
+                            fb = objective(bb);
+                            if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) { cc = aa; fc = fa; }
+                        }
+                        if (std::isnan(pt)) pt = bb;   // iterations exhausted
+                    }
+                }
+                if (side == 0) upper[i] = pt;
+                else lower[i] = pt;
+                evals[i] = (side == 0 ? 0 : evals[i]) + n;
+            }
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        g_orc_err = ex.what();
+        return 1;
+    }
 }
 
 double orc_prior_delta(int type, double variance, double g, double h, double beta) {
