@@ -146,14 +146,15 @@ __global__ __launch_bounds__(256) void kGram(int64_t R, int m, int nt, const dou
     }
 }
 
-// out[i][j] += sign * sum over splits of the partial of (i, j), read from the tile
-// pair that holds it (the upper one when the tiles of i and j differ).
+// out[i][j] += sign * sum over splits of the partial of (min(i,j), max(i,j)): every
+// such entry lies in an upper tile, and reading one entry for both halves keeps the
+// result exactly symmetric (inside a diagonal tile (i,j) and (j,i) are rounded apart).
 __global__ __launch_bounds__(BLOCK) void kGramReduce(int m, int splits, const double* __restrict__ part, double sign,
                                                      double* __restrict__ out) {
     const int64_t mm = (int64_t)m * m;
     for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < mm; t += (int64_t)gridDim.x * BLOCK) {
         int i = (int)(t / m), j = (int)(t % m);
-        if (i / GT > j / GT) { const int x = i; i = j; j = x; }
+        if (i > j) { const int x = i; i = j; j = x; }
         double s = 0.0;
         for (int p = 0; p < splits; ++p) s += part[((size_t)p * m + i) * m + j];
         out[t] += sign * s;
