@@ -50,7 +50,8 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=3, help="CCD cycles of the CPU baseline sample (about 10 s)")
-    ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic"], default="poisson",
+    ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic", "variance"],
+                    default="poisson",
                     help="poisson: the batched-fit headline line (default); serving: the request-mix server; "
                          "lr / cox / clr-cv: the other benchmark workloads")
     ap.add_argument("--requests", type=int, default=40, help="serving: requests in the mix")
@@ -68,6 +69,8 @@ def main():
         return run_serving(args)
     if args.workload == "traffic":
         return run_traffic(args)
+    if args.workload == "variance":
+        return run_variance(args)
     if args.workload != "poisson":
         return run_other(args)
 
@@ -457,6 +460,80 @@ def run_other(args):
         "speedup_vs_plain": (B / step) * plain_pc * it_ref,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": launch_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_variance(args):
+    """Variance and profile-likelihood consumers on the logistic workload (1e5 rows x
+    1,001 columns, FP64): the Fisher information of every column at the fitted beta,
+    and profile-likelihood 95% limits of 16 covariates left out of the prior (32 bound
+    searches advanced together, one batched fit per round), against the same limits
+    taken one column at a time and against the CPU oracle on a sample of columns."""
+    from cyclops_amd import _abi
+    from cyclops_amd import data as D
+    from cyclops_amd import engine as E
+    from cyclops_amd.control import create_control, create_prior
+
+    if E.device_count() < 1:
+        raise SystemExit("no HIP device")
+    d, _ = D.make_config(0, scale=args.scale)
+    log(f"generated logistic workload K={d.K} J={d.J} nnz={d.nnz}")
+    n_prof = 16
+    cols = list(range(1, 1 + n_prof))   # column 0 is the intercept (never penalised)
+    prior = create_prior("laplace", 0.1, exclude=cols)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-6)
+    thr = 3.841458820694124 / 2.0   # qchisq(0.95, 1) / 2
+    eng = E.Engine(d, _abi.FP64)
+    b, r = eng.fit(prior, ctl, mode="resident")
+    log(f"mode: {r['iterations']} cycles, loglik {r['log_likelihood']:.4f}")
+    # Fisher information of every column (R's vcov)
+    allc = list(range(d.J))
+    eng.fisher_information(b, allc[:64])   # warm-up (module load, first allocations)
+    t0 = time.perf_counter()
+    F = eng.fisher_information(b, allc)
+    t_fis = time.perf_counter() - t0
+    m = d.J
+    tiles = (m + 31) // 32
+    flops = 2.0 * d.K * (tiles * (tiles + 1) // 2) * 32 * 32   # FMAs of the upper tiles, x2
+    log(f"fisher information {m} x {m}: {t_fis * 1e3:.1f} ms ({flops / t_fis / 1e12:.2f} TFLOP/s FP64)")
+    # profile limits: all columns together, then one column at a time
+    t0 = time.perf_counter()
+    lo, up, ev = eng.profile(prior, ctl, b, cols, thr)
+    t_all = time.perf_counter() - t0
+    log(f"profile {n_prof} columns together: {t_all:.2f}s, {int(ev.sum())} evaluations")
+    t0 = time.perf_counter()
+    seq = [eng.profile(prior, ctl, b, [c], thr) for c in cols]
+    t_seq = time.perf_counter() - t0
+    lo_s = np.array([x[0][0] for x in seq])
+    up_s = np.array([x[1][0] for x in seq])
+    log(f"profile one column at a time: {t_seq:.2f}s")
+    eng.close()
+    cpu = None
+    if not args.skip_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        lo_o, up_o, ev_o = oracle.profile(d, prior, ctl, b, cols[:1], threshold=thr)
+        t_cpu = time.perf_counter() - t0
+        log(f"cpu oracle: one column {t_cpu:.1f}s ({int(ev_o.sum())} evaluations)")
+        cpu = {"value": 1.0 / t_cpu, "unit": "columns/s", "cores": 1, "kind": "port",
+               "sample": f"profile limits of the first of the {n_prof} columns in FP64 on one host core "
+                         f"({t_cpu:.1f}s, {int(ev_o.sum())} evaluations); "
+                         f"device vs oracle on it: lower {abs(lo[0] - lo_o[0]):.1e}, upper {abs(up[0] - up_o[0]):.1e}"}
+    out = {
+        "metric": "profile-likelihood confidence limits per second (confint) and Fisher information time (vcov)",
+        "value": n_prof / t_all, "unit": "columns/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t_all * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded generator, no dataset files)",
+        "config": {"workload": "lr_K1e5_P1e3_density0.01_laplace0.1_16_unpenalised_columns_profiled",
+                   "K": d.K, "J": d.J, "nnz": d.nnz, "profiled_columns": n_prof, "level": 0.95,
+                   "evaluations": int(ev.sum())},
+        "one_column_at_a_time": {"seconds": t_seq, "columns_per_sec": n_prof / t_seq,
+                                 "max_abs_diff_of_limits": float(max(np.max(np.abs(lo - lo_s)),
+                                                                     np.max(np.abs(up - up_s))))},
+        "fisher_information": {"columns": m, "ms": t_fis * 1e3, "tflops_fp64": flops / t_fis / 1e12,
+                               "peak_tflops_fp64": 78.6, "symmetric": bool(np.array_equal(F, F.T))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
