@@ -107,7 +107,7 @@ class CvArgs(C.Structure):
         ("lower_limit", C.c_double),
         ("upper_limit", C.c_double),
         ("selector", C.c_int32),
-        ("pad_", C.c_int32),
+        ("warm_start", C.c_int32),
         ("seed", C.c_int64),
     ]
 

@@ -33,7 +33,7 @@ class Control:
     upper_limit: float = 20.0
     seed: int = 1
     selector: str = "auto"               # auto | byRow | byPid
-    reset_coefficients: bool = True
+    reset_coefficients: bool = False     # R's default: cross-validation fits warm-start in a chain
 
 
 def create_prior(prior_type="none", variance=1.0, exclude=(), use_cross_validation=False) -> Prior:

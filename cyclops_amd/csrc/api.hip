@@ -233,13 +233,14 @@ void assignFolds(int64_t K, const int32_t* pidIn, int fold, int selector, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Grid-search cross-validation, cold start (resetCoefficients = TRUE):
-// GridSearchCrossValidationDriver::doCrossValidationLoop (:96-138) with
-// computeGridPoint (:38-44), AbstractCrossValidationDriver::doCrossValidationStep
-// (:131-248), computePointEstimate (:250-261), findMax (:234-244) and
-// resetForOptimal (:31-42) followed by the final fit (CcdInterface.cpp:777-789).
-// All fold x grid-point fits are independent, so they run as one batched launch
-// when the engine supports it, else one resident fit after another.
+// Grid-search cross-validation: GridSearchCrossValidationDriver::doCrossValidationLoop
+// (:96-138) with computeGridPoint (:38-44), AbstractCrossValidationDriver::
+// doCrossValidationStep (:131-248), computePointEstimate (:250-261), findMax
+// (:234-244) and resetForOptimal (:31-42) followed by the final fit
+// (CcdInterface.cpp:777-789). Cold start (resetCoefficients = TRUE): the fold x
+// grid-point fits are independent and run as one batched launch. Warm start (R's
+// default): one chain in the reference's task order, each fit a persistent launch
+// starting from the previous fit's coefficients.
 // ---------------------------------------------------------------------------
 static void crossValidate(EngineBase& eng, const ccd_fit_args& a, const ccd_cv_args& cv, double* gridValues,
                           double* predLL, double* betaOut, ccd_fit_result& fin, ccd_cv_result& out) {
@@ -268,6 +269,53 @@ static void crossValidate(EngineBase& eng, const ccd_fit_args& a, const ccd_cv_a
         }
     }
     const int nFits = cv.grid_size * cv.fold;
+    if (cv.warm_start) {
+        std::vector<double> beta(J, 0.0), w(K), comp(K);
+        if (eng.hasOffset()) beta[0] = 1.0;
+        double bestValue = -std::numeric_limits<double>::infinity(), bestPoint = points[0];
+        for (int step = 0; step < cv.grid_size; ++step) {
+            double total = 0.0;
+            int count = 0;
+            for (int f = 0; f < cv.fold; ++f) {
+                for (int64_t k = 0; k < K; ++k) {
+                    w[k] = rowFold[k] == f ? 0.0 : 1.0;
+                    comp[k] = 1.0 - w[k];
+                }
+                ccd_fit_args fa = a;
+                fa.weights = w.data();
+                fa.variance = points[step];
+                ccd_fit_result r{};
+                eng.fitResident(fa, beta.data(), r);   // starts from the previous fit's beta
+                double v = std::numeric_limits<double>::quiet_NaN();
+                if (r.return_flag == CCD_SUCCESS) {
+                    eng.setWeights(w.data(), true);
+                    eng.computeXBeta(beta.data());
+                    eng.remainingStats(true);
+                    v = eng.predLogLik(comp.data());
+                } else {   // cold start for stability (:224-226)
+                    std::fill(beta.begin(), beta.end(), 0.0);
+                    if (eng.hasOffset()) beta[0] = 1.0;
+                }
+                if (predLL) predLL[step * cv.fold + f] = v;
+                if (v == v) { total += v; ++count; }
+            }
+            const double value = total / (double)count;
+            gridValues[step] = value;
+            if (step == 0 || value > bestValue) { bestValue = value; bestPoint = points[step]; }
+        }
+        std::fill(beta.begin(), beta.end(), 0.0);   // resetForOptimal: cold start
+        if (eng.hasOffset()) beta[0] = 1.0;
+        ccd_fit_args fa = a;
+        fa.weights = nullptr;
+        fa.variance = bestPoint;
+        eng.fitResident(fa, beta.data(), fin);
+        std::copy(beta.begin(), beta.end(), betaOut);
+        out.best_variance = bestPoint;
+        out.best_value = bestValue;
+        out.n_fits = nFits;
+        out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return;
+    }
     std::vector<double> var(nFits), betas((size_t)nFits * J, 0.0);
     std::vector<int32_t> held(nFits);
     for (int step = 0; step < cv.grid_size; ++step)

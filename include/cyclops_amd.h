@@ -245,9 +245,14 @@ int ccd_profile(ccd_handle h, const ccd_fit_args* args, const double* beta, cons
  * CrossValidationSelector (drivers/CrossValidationSelector.cpp:19-189) with
  * GridSearchCrossValidationDriver (drivers/GridSearchCrossValidationDriver.cpp:38-44,
  * 96-138,234-244) and AbstractCrossValidationDriver::doCrossValidationStep /
- * resetForOptimal (AbstractCrossValidationDriver.cpp:31-42,131-248), cold start
- * (resetCoefficients = TRUE). All fold x grid-point fits run as one batched
- * launch; the final refit at the optimum runs in FP64. */
+ * resetForOptimal (AbstractCrossValidationDriver.cpp:31-42,131-248).
+ * warm_start = 0 (resetCoefficients = TRUE): every fit starts from zero, so all
+ *   fold x grid-point fits are independent and run as one batched launch.
+ * warm_start = 1 (resetCoefficients = FALSE, R's default, one thread): the fits run
+ *   in the reference's task order (grid point, then fold), each starting from the
+ *   previous fit's coefficients, and a fit that fails restarts the chain from zero
+ *   (:201-203, 224-226); one persistent launch per fit.
+ * The final refit at the optimum starts from zero. */
 #define CCD_SELECT_BY_ROW  0
 #define CCD_SELECT_BY_PID  1
 typedef struct ccd_cv_args {
@@ -256,7 +261,7 @@ typedef struct ccd_cv_args {
     double  lower_limit;      /* smallest variance */
     double  upper_limit;      /* largest variance */
     int32_t selector;         /* CCD_SELECT_BY_ROW / CCD_SELECT_BY_PID */
-    int32_t pad_;
+    int32_t warm_start;       /* 1 = resetCoefficients FALSE (chain), 0 = cold start (batched) */
     int64_t seed;             /* mt19937 seed of the fold permutation */
 } ccd_cv_args;
 typedef struct ccd_cv_result {

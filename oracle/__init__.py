@@ -245,6 +245,7 @@ def cross_validate(data, prior, control, selector):
     cv.lower_limit = control.lower_limit
     cv.upper_limit = control.upper_limit
     cv.selector = selector
+    cv.warm_start = 0 if control.reset_coefficients else 1
     cv.seed = control.seed
     grid = np.zeros(control.grid_size)
     pred = np.zeros(control.grid_size * control.fold)

@@ -1003,8 +1003,10 @@ int orc_cv_assign_folds(int64_t K, const int32_t* pid, int32_t fold, int32_t sel
     return 0;
 }
 
-/* Grid-search cross-validation, single-threaded reference semantics with cold
- * start (resetCoefficients = TRUE): AbstractCrossValidationDriver.cpp:131-248. */
+/* Grid-search cross-validation, single-threaded reference semantics
+ * (AbstractCrossValidationDriver.cpp:131-248): one CCD runs every fit in task order;
+ * cold start resets beta before each fit (:201-203), a warm start keeps the previous
+ * fit's beta; a failed fit resets beta either way (:224-226). */
 int orc_cross_validate(void* hv, const ccd_fit_args* a, const ccd_cv_args* cv, double* gridValues,
                        double* predLL, double* beta, ccd_fit_result* r, double* bestVariance) {
     try {
@@ -1032,7 +1034,7 @@ int orc_cross_validate(void* hv, const ccd_fit_args* a, const ccd_cv_args* cv, d
                 std::vector<double> w(d.K);
                 for (int64_t k = 0; k < d.K; ++k) w[k] = (rowFold[k] == f) ? 0.0 : 1.0;
                 ccd.setWeights(w.data());
-                ccd.resetBeta();
+                if (!cv->warm_start) ccd.resetBeta();
                 ccd.update(*a);
                 double v = std::numeric_limits<double>::quiet_NaN();
                 if (ccd.lastFlag == CCD_SUCCESS) {

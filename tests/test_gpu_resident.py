@@ -189,13 +189,15 @@ def test_grouped_multi_workgroup_matches_oracle(E, model):
     assert abs(r["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
 
 
+@pytest.mark.parametrize("reset", [True, False])
 @pytest.mark.parametrize("model", ["lr", "clr"])
-def test_cross_validation_matches_oracle(E, model):
+def test_cross_validation_matches_oracle(E, model, reset):
+    """Cold start (all fits batched) and R's default warm-start chain."""
     d = _cases.ALL[model]()
-    print(f"[case] cross-validation {model}", flush=True)
+    print(f"[case] cross-validation {model} reset_coefficients={reset}", flush=True)
     prior = create_prior("laplace", 1.0)
     ctl = create_control(convergence_type="mittal", tolerance=1e-8, fold=4, grid_size=5,
-                         lower_limit=0.01, upper_limit=5.0, seed=123)
+                         lower_limit=0.01, upper_limit=5.0, seed=123, reset_coefficients=reset)
     selector = _abi.SELECT_BY_PID if model == "clr" else _abi.SELECT_BY_ROW
     ro = oracle.cross_validate(d, prior, ctl, selector)
     eng = E.Engine(d, 64)
