@@ -19,6 +19,7 @@ constexpr int WAVES = BLOCK / 64;
 constexpr int ITEMS = 4;                 // consecutive entries per thread in ordered passes
 constexpr int TILE = BLOCK * ITEMS;
 constexpr int MAX_RED_BLOCKS = BLOCK;    // reduction kernels: one partial per thread of the last block
+constexpr int MAX_GRAD_BLOCKS = 4 * BLOCK;   // column gradient passes of the plain path
 
 // Per-block partial of a column reduction.
 //   g, h : gradient / Hessian numerators

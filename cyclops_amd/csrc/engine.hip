@@ -252,7 +252,7 @@ public:
             scanParts = dalloc<Seg2>((capN + TILE - 1) / TILE + 1, bytes);
         }
         XjY = dalloc<double>(D->J, bytes);
-        parts = dalloc<Partial>(MAX_RED_BLOCKS, bytes);
+        parts = dalloc<Partial>(std::max(MAX_RED_BLOCKS, MAX_GRAD_BLOCKS), bytes);
         counter = dalloc<unsigned>(1, bytes);
         dOut = dalloc<double>(8, bytes);
         if (arena) hOut = arena->host;
@@ -497,7 +497,9 @@ public:
     void gradHessAsync(int j, bool useW) {
         const bool W = useW && weighted;
         const Col<Real> c = col(j);
-        int nb = (int)std::min<int64_t>(MAX_RED_BLOCKS, std::max<int64_t>(1, (c.n + TILE * 2 - 1) / (TILE * 2)));
+        const bool indep = D->model == M_LOGISTIC || D->model == M_POISSON;
+        int nb = (int)std::min<int64_t>(indep ? MAX_GRAD_BLOCKS : MAX_RED_BLOCKS,
+                                        std::max<int64_t>(1, (c.n + TILE * 2 - 1) / (TILE * 2)));
         const int64_t per = (c.n + nb - 1) / nb;
         const double xjy = hXjY[j];
         const int m = D->model;
@@ -578,8 +580,30 @@ public:
     }
     void updateXBeta(double delta, int j, bool useW) override {
         CCD_HIP(hipSetDevice(D->device));
+        if (timing) CCD_HIP(hipEventRecord(ev0, s));
         updateXBetaAsync(delta, j, useW);
+        if (timing) CCD_HIP(hipEventRecord(ev1, s));
         CCD_HIP(hipStreamSynchronize(s));
+        if (timing && delta != 0.0 && nonZeros(j) > 0) {
+            float ms = 0.f;
+            CCD_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            counters.deviceMs += ms;
+            counters.launches += 1;
+            counters.bytes += updateBytes(j);
+        }
+    }
+    // Compulsory bytes of one update of column j: row ids (+values) and a
+    // read-modify-write of xb plus the write of exp(xb) per entry (grouped models:
+    // pid and the per-pid denominator, FP64).
+    double updateBytes(int j) const {
+        const double n = (double)nonZeros(j);
+        const int f = D->fmt[j];
+        const double s_ = sizeof(Real);
+        double b = n * 3.0 * s_;
+        if (f == F_INDICATOR || f == F_SPARSE) b += n * 4.0;
+        if (f == F_SPARSE || f == F_DENSE) b += n * s_;
+        if (D->model == M_CLR || D->model == M_COX) b += n * (4.0 + 16.0);
+        return b;
     }
 
     // ---- reductions ---------------------------------------------------------

@@ -174,31 +174,46 @@ __global__ __launch_bounds__(BLOCK) void kGradIndep(Col<Real> c, const Real* __r
                                                    unsigned* counter, double xjy, double* out,
                                                    int64_t perBlock) {
     constexpr bool IND = isIndicatorFmt<FMT>();
+    constexpr int U = 4;   // entries whose gathers are in flight together
     const int64_t lo = (int64_t)blockIdx.x * perBlock;
     const int64_t hi = lo + perBlock < c.n ? lo + perBlock : c.n;
     double g = 0.0, h = 0.0;
-    for (int64_t e = lo + threadIdx.x; e < hi; e += BLOCK) {
-        const int64_t k = rowOf<Real, FMT>(c, e);
-        const Real x = xOf<Real, FMT>(c, e);
-        const Real ee = ex[k];
-        Real gi, hh;
-        if constexpr (MODEL == M_LOGISTIC) {
-            const Real D = Real(1) + ee;
-            gi = ee * x / D;
-            if constexpr (IND) hh = gi * (Real(1) - gi);
-            else hh = ee * x * x / D - gi * gi;
-        } else {
-            gi = ee * x;
-            if constexpr (IND) hh = gi;
-            else hh = gi * x;
+    for (int64_t e0 = lo + threadIdx.x; e0 < hi; e0 += U * BLOCK) {
+        int64_t k[U];
+        Real x[U], ee[U], w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = e0 + (int64_t)u * BLOCK;
+            k[u] = e < hi ? rowOf<Real, FMT>(c, e) : -1;
+            x[u] = e < hi ? xOf<Real, FMT>(c, e) : Real(0);
         }
-        if constexpr (W) {
-            const Real w = kw[k];
-            gi *= w;
-            hh *= w;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ee[u] = k[u] >= 0 ? ex[k[u]] : Real(0);
+            if constexpr (W) w[u] = k[u] >= 0 ? kw[k[u]] : Real(0);
         }
-        g += (double)gi;
-        h += (double)hh;
+        // accumulated in entry order, as one entry per step would
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k[u] < 0) break;
+            Real gi, hh;
+            if constexpr (MODEL == M_LOGISTIC) {
+                const Real D = Real(1) + ee[u];
+                gi = ee[u] * x[u] / D;
+                if constexpr (IND) hh = gi * (Real(1) - gi);
+                else hh = ee[u] * x[u] * x[u] / D - gi * gi;
+            } else {
+                gi = ee[u] * x[u];
+                if constexpr (IND) hh = gi;
+                else hh = gi * x[u];
+            }
+            if constexpr (W) {
+                gi *= w[u];
+                hh *= w[u];
+            }
+            g += (double)gi;
+            h += (double)hh;
+        }
     }
     double v[2] = {g, h};
     blockReduce<2>(v);
