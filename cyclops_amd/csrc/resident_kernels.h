@@ -768,9 +768,9 @@ __device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, 
 
 // Breslow risk sets of this workgroup's pids, both scans, with the cross-workgroup carries.
 // One fit per team: the same rescan with the workgroup's pids split into 4
-// contiguous wave ranges, each walked 64 consecutive pids per step (coalesced
-// loads, a wave segmented scan per step), then carried across waves and, through
-// the slots, across workgroups.
+// contiguous wave ranges, each walked in chunks of 8 consecutive pids per lane
+// (a register scan per lane, one wave scan of the lane totals per chunk), then
+// carried across waves and, through the slots, across workgroups.
 template <typename Real, int BP>
 __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
                                                Shared<BP>& s) {
@@ -792,67 +792,106 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
         r.pad = 0;
         return r;
     };
-    constexpr int SU = 8;   // steps whose loads are issued together
-    // forward term of pid p (identity outside [a, b))
-    auto fwdVal = [&](int64_t p, double d, int sp, int sq) {
-        return Seg2{d, 0.0, (p > 0 && sp != sq) ? 1 : 0, 0};
+    // Chunks of CH consecutive pids per lane (64 * CH per wave): each lane scans its
+    // pids in registers and ONE wave scan of the lane totals joins them, instead of
+    // a wave scan per 64 pids.
+    constexpr int CH = 8;
+    constexpr int CW = 64 * CH;
+    // wave exclusive prefix of the lane totals and the wave's total
+    auto laneJoin = [&](const Seg2& tot, Seg2& excl, Seg2& all) {
+        const Seg2 inc = waveSegScanIncl(tot);
+        excl.a = __shfl_up(inc.a, 1, 64);
+        excl.b = __shfl_up(inc.b, 1, 64);
+        excl.f = __shfl_up(inc.f, 1, 64);
+        excl.pad = 0;
+        if (lane == 0) excl = z;
+        all = bcast63(inc);
     };
-    // walks [a, b) forward; visit(p, inclusive-in-step value, valid) per lane, step
-    // totals combined into `run` by the caller's rule
+    // walks [a, b) forward: D = segmented prefix of denom; a pid whose stratum
+    // differs from the previous pid's starts a new segment
     auto fwdWalk = [&](Seg2 run, bool store) {
-        for (int64_t base = a; base < b; base += 64 * SU) {
-            double dv[SU];
-            int sp[SU], sq[SU];
+        for (int64_t base = a; base < b; base += CW) {
+            const int64_t p0 = base + (int64_t)lane * CH;
+            double dv[CH];
+            int sg[CH];
+            const int sPrev = (p0 > 0 && p0 < b) ? P.pseg[p0 - 1] : 0;
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int64_t p = base + u * 64 + lane;
-                const bool ok = p < b;
-                dv[u] = ok ? P.denom[p] : 0.0;
-                sp[u] = ok ? P.pseg[p] : 0;
-                sq[u] = (ok && p > 0) ? P.pseg[p - 1] : 0;
+            for (int i = 0; i < CH; ++i) {
+                const bool ok = p0 + i < b;
+                dv[i] = ok ? P.denom[p0 + i] : 0.0;
+                sg[i] = ok ? P.pseg[p0 + i] : 0;
             }
+            Seg2 acc = z;
+            double inc[CH];
+            bool started[CH];
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int64_t p = base + u * 64 + lane;
-                const bool ok = p < b;
-                const Seg2 inc = waveSegScanIncl(ok ? fwdVal(p, dv[u], sp[u], sq[u]) : z);
-                if (store && ok) P.Dacc[p] = segCombine(run, inc).a;
-                run = segCombine(run, bcast63(inc));
+            for (int i = 0; i < CH; ++i) {
+                const int64_t p = p0 + i;
+                const int prev = i == 0 ? sPrev : sg[i - 1];
+                const bool st = p < b && p > 0 && sg[i] != prev;
+                acc = segCombine(acc, Seg2{dv[i], 0.0, st ? 1 : 0, 0});
+                inc[i] = acc.a;
+                started[i] = acc.f != 0;
             }
+            Seg2 excl, all;
+            laneJoin(acc, excl, all);
+            if (store) {
+                const Seg2 pre = segCombine(run, excl);
+#pragma unroll
+                for (int i = 0; i < CH; ++i)
+                    if (p0 + i < b) P.Dacc[p0 + i] = started[i] ? inc[i] : pre.a + inc[i];
+            }
+            run = segCombine(run, all);
         }
         return run;
     };
-    // walks [a, b) backward (descending pids)
+    // walks [a, b) backward (descending pids): R1, R2 = reverse segmented sums of
+    // nE/D and nE/D^2; the last pid of a stratum (or of all pids) starts a segment
     auto bwdWalk = [&](Seg2 run, bool store) {
-        for (int64_t top = b - 1; top >= a; top -= 64 * SU) {
-            double nv[SU], dv[SU];
-            int sp[SU], sn[SU];
+        for (int64_t top = b - 1; top >= a; top -= CW) {
+            const int64_t p0 = top - (int64_t)lane * CH;   // this lane's pids: p0, p0 - 1, ...
+            double nv[CH], dv[CH];
+            int sg[CH];
+            const int sNext = (p0 >= a && p0 + 1 < N) ? P.pseg[p0 + 1] : 0;
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int64_t p = top - u * 64 - lane;
+            for (int i = 0; i < CH; ++i) {
+                const int64_t p = p0 - i;
                 const bool ok = p >= a;
-                nv[u] = ok ? P.nW[p] : 0.0;
-                dv[u] = ok ? P.Dacc[p] : 1.0;
-                sp[u] = ok ? P.pseg[p] : 0;
-                sn[u] = ok ? P.pseg[p + 1] : 0;
+                nv[i] = ok ? P.nW[p] : 0.0;
+                dv[i] = ok ? P.Dacc[p] : 1.0;
+                sg[i] = ok ? P.pseg[p] : 0;
             }
+            Seg2 acc = z;
+            double i1[CH], i2[CH];
+            bool started[CH];
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int64_t p = top - u * 64 - lane;
-                const bool ok = p >= a;
+            for (int i = 0; i < CH; ++i) {
+                const int64_t p = p0 - i;
                 Seg2 v = z;
-                if (ok) {
-                    const double r = nv[u] == 0.0 ? 0.0 : nv[u] / dv[u];
-                    v = Seg2{r, nv[u] == 0.0 ? 0.0 : r / dv[u], (p == N - 1 || sn[u] != sp[u]) ? 1 : 0, 0};
+                if (p >= a) {
+                    const int next = i == 0 ? sNext : sg[i - 1];
+                    const double r = nv[i] == 0.0 ? 0.0 : nv[i] / dv[i];
+                    v = Seg2{r, nv[i] == 0.0 ? 0.0 : r / dv[i], (p == N - 1 || next != sg[i]) ? 1 : 0, 0};
                 }
-                const Seg2 inc = waveSegScanIncl(v);
-                if (store && ok) {
-                    const Seg2 o = segCombine(run, inc);
-                    P.R1[p] = o.a;
-                    P.R2[p] = o.b;
-                }
-                run = segCombine(run, bcast63(inc));
+                acc = segCombine(acc, v);
+                i1[i] = acc.a;
+                i2[i] = acc.b;
+                started[i] = acc.f != 0;
             }
+            Seg2 excl, all;
+            laneJoin(acc, excl, all);
+            if (store) {
+                const Seg2 pre = segCombine(run, excl);
+#pragma unroll
+                for (int i = 0; i < CH; ++i) {
+                    const int64_t p = p0 - i;
+                    if (p >= a) {
+                        P.R1[p] = started[i] ? i1[i] : pre.a + i1[i];
+                        P.R2[p] = started[i] ? i2[i] : pre.b + i2[i];
+                    }
+                }
+            }
+            run = segCombine(run, all);
         }
         return run;
     };
