@@ -380,10 +380,29 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n
             hasPrev = true;
         }
     }
-    for (int64_t e = n0; e < n1; e += U) {
+    if constexpr (!GROUPED) {
+        // the next batch's row ids and values are requested before this batch's
+        // gathers, so their latencies overlap
         Batch<Real, MODEL, BP, U> bt;
-        loadBatch<Real, MODEL, BP, FMT, U>(P, j, e, n1, on, bt);
-        accumBatch<Real, MODEL, BP, U>(bt, on, acc, prevSeg, hasPrev);
+        if (n0 < n1) loadRows<Real, MODEL, BP, FMT, U>(P, j, n0, n1, bt);
+        for (int64_t e = n0; e < n1; e += U) {
+            Batch<Real, MODEL, BP, U> nx;
+            if (e + U < n1) loadRows<Real, MODEL, BP, FMT, U>(P, j, e + U, n1, nx);
+            loadState<Real, MODEL, BP, U>(P, on, bt);
+            accumBatch<Real, MODEL, BP, U>(bt, on, acc, prevSeg, hasPrev);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bt.kk[u] = nx.kk[u];
+                bt.xx[u] = nx.xx[u];
+                bt.val[u] = nx.val[u];
+            }
+        }
+    } else {
+        for (int64_t e = n0; e < n1; e += U) {
+            Batch<Real, MODEL, BP, U> bt;
+            loadBatch<Real, MODEL, BP, FMT, U>(P, j, e, n1, on, bt);
+            accumBatch<Real, MODEL, BP, U>(bt, on, acc, prevSeg, hasPrev);
+        }
     }
 }
 
