@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # plus WRITE_SIZE x 1 (the factors a 1 GiB device copy reads back), summed over the
 # step's launches and divided by the bytes the kernel counts.  The measurement and its
 # commands are in profiles/kFindMode_poisson_K1e6_J2e4_batch64_traffic.md.
-TRAFFIC_PER_ALGORITHMIC_BYTE = 1.2335
+TRAFFIC_PER_ALGORITHMIC_BYTE = 1.2422
 
 
 def log(msg):
