@@ -532,6 +532,16 @@ int ccd_profile(ccd_handle h, const ccd_fit_args* args, const double* beta, cons
     });
 }
 
+int ccd_profile_likelihood(ccd_handle h, const ccd_fit_args* args, const double* beta, int32_t index,
+                           const double* points, int32_t n, int include_penalty, double* values) {
+    return guard([&] {
+        if (!args || !beta || (n > 0 && (!points || !values))) throw std::invalid_argument("null argument");
+        if (n < 0) throw std::invalid_argument("n must be >= 0");
+        ccd::checkArgs(E(h), *args);
+        ccd::profileLikelihood(E(h), *args, beta, index, points, n, include_penalty != 0, values);
+    });
+}
+
 int ccd_cross_validate(ccd_handle h, const ccd_fit_args* common, const ccd_cv_args* cv, double* grid_values,
                        double* pred_loglik, double* beta, ccd_fit_result* final_result, ccd_cv_result* cv_result) {
     return guard([&] {

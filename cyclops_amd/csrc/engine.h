@@ -119,6 +119,11 @@ void profileModel(EngineBase& eng, const ccd_fit_args& a, const double* beta, co
                   double threshold, bool includePenalty, bool overrideNoRegularization, double* lower,
                   double* upper, int32_t* evaluations);
 
+// Profile log-likelihood at grid points (CcdInterface::evaluateProfileModel,
+// CcdInterface.cpp:527-607): one batched fit over the points (profile.hip).
+void profileLikelihood(EngineBase& eng, const ccd_fit_args& a, const double* beta, int index,
+                       const double* points, int n, bool includePenalty, double* values);
+
 // Fold assignment of CrossValidationSelector (drivers/CrossValidationSelector.cpp:19-189).
 void assignFolds(int64_t K, const int32_t* pid, int fold, int selector, int64_t seed, int32_t* rowFold);
 

@@ -242,4 +242,45 @@ void profileModel(EngineBase& eng, const ccd_fit_args& a, const double* beta, co
     }
 }
 
+void profileLikelihood(EngineBase& eng, const ccd_fit_args& a, const double* beta, int index,
+                       const double* points, int n, bool includePenalty, double* values) {
+    if (a.weights) throw std::invalid_argument("profiling takes no row weights");
+    const int J = eng.J();
+    if (index < 0 || index >= J) throw std::invalid_argument("Variable " + std::to_string(index) + " not found.");
+    if ((eng.hasOffset() && index == 0) || (a.fixed && a.fixed[index]))
+        throw std::invalid_argument("Cannot profile a fixed coefficient");
+    if (n <= 0) return;
+    ccd_fit_args common = a;
+    common.weights = nullptr;
+    // the column held at 0, from beta (CcdInterface.cpp:570)
+    std::vector<double> start(beta, beta + J);
+    if (eng.hasOffset()) start[0] = 1.0;
+    start[index] = 0.0;
+    {
+        double var = a.variance;
+        int32_t fix = index;
+        ccd_batch_args one{};
+        one.n_fits = 1;
+        one.variances = &var;
+        one.fixed_index = &fix;
+        ccd_fit_result r{};
+        eng.fitBatch(common, one, start.data(), &r, false);
+    }
+    // every point from that fit, all in one batch (OptimizationProfile::objective
+    // with max = threshold = 0, :238-249)
+    std::vector<double> betas((size_t)n * J), var(n, a.variance);
+    std::vector<int32_t> fix(n, index);
+    std::vector<ccd_fit_result> res(n);
+    for (int i = 0; i < n; ++i) {
+        std::copy(start.begin(), start.end(), betas.begin() + (size_t)i * J);
+        betas[(size_t)i * J + index] = points[i];
+    }
+    ccd_batch_args ba{};
+    ba.n_fits = n;
+    ba.variances = var.data();
+    ba.fixed_index = fix.data();
+    eng.fitBatch(common, ba, betas.data(), res.data(), false);
+    for (int i = 0; i < n; ++i) values[i] = res[i].log_likelihood + (includePenalty ? res[i].log_prior : 0.0);
+}
+
 }  // namespace ccd

@@ -70,6 +70,7 @@ def lib():
         L.ccd_fisher_information.argtypes = [vp, dp, C.POINTER(i32), i32, dp]
         L.ccd_profile.argtypes = [vp, C.POINTER(_abi.FitArgs), dp, C.POINTER(i32), i32, C.c_double, C.c_int,
                                   C.c_int, dp, dp, C.POINTER(i32)]
+        L.ccd_profile_likelihood.argtypes = [vp, C.POINTER(_abi.FitArgs), dp, i32, dp, i32, C.c_int, dp]
         L.ccd_get_kernel_stats.argtypes = [vp, C.POINTER(_abi.KernelStats)]
         L.ccd_reset_kernel_stats.argtypes = [vp]
         L.ccd_estimate_footprint.argtypes = [C.POINTER(_abi.DataDesc), dp]
@@ -301,6 +302,17 @@ class Engine:
                                 _d(lo), _d(up), ev.ctypes.data_as(C.POINTER(C.c_int32))))
         return lo, up, ev
 
+    def profile_likelihood(self, prior, control, beta, index, points, include_penalty=True):
+        """Profile log-likelihood of column `index` at `points`: cyclopsGetProfileLikelihood
+        (CcdInterface::evaluateProfileModel), all points as one batched fit."""
+        hold = FitArgsHolder(self.data.J, prior, control, intercept_index=intercept_index(self.data))
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        x = np.ascontiguousarray(points, dtype=np.float64)
+        out = np.zeros(len(x))
+        check(lib().ccd_profile_likelihood(self.h, C.byref(hold.args), _d(b), int(index), _d(x), len(x),
+                                           int(bool(include_penalty)), _d(out)))
+        return out
+
     def kernel_stats(self):
         st = _abi.KernelStats()
         check(lib().ccd_get_kernel_stats(self.h, C.byref(st)))
@@ -365,6 +377,13 @@ def confint(engine, prior, control, beta, parm=None, level=0.95, override_no_reg
     lo, up, _ = engine.profile(prior, control, beta, cols, chi2.ppf(level, 1) / 2.0, include_penalty,
                                override_no_regularization)
     return np.column_stack([np.array(cols, dtype=np.float64), lo, up])
+
+
+def get_cyclops_profile_log_likelihood(engine, prior, control, beta, parm, x, include_penalty=True):
+    """getCyclopsProfileLogLikelihood (R/ModelFit.R:816-826): the profile log-likelihood
+    of one covariate at each value of x."""
+    col = _covariates(engine.data, [parm])[0]
+    return engine.profile_likelihood(prior, control, beta, col, x, include_penalty)
 
 
 def assign_folds(K, pid, fold, selector, seed):

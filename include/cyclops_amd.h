@@ -241,6 +241,16 @@ int ccd_profile(ccd_handle h, const ccd_fit_args* args, const double* beta, cons
                 double threshold, int include_penalty, int override_no_regularization,
                 double* lower, double* upper, int32_t* evaluations);
 
+/* Profile log-likelihood on a grid: CcdInterface::evaluateProfileModel
+ * (CcdInterface.cpp:527-607), R's getCyclopsProfileLogLikelihood (R/ModelFit.R:816-826).
+ * values[i] = log-likelihood (plus the log prior when include_penalty) of the fit
+ * with column `index` held at points[i]. As the reference, the column is first held
+ * at 0 starting from beta (:570); every point's fit then starts from that fit, as
+ * the per-thread clones made after it do (:572-577). All points run as one batched
+ * fit. args->weights must be NULL. */
+int ccd_profile_likelihood(ccd_handle h, const ccd_fit_args* args, const double* beta, int32_t index,
+                           const double* points, int32_t n, int include_penalty, double* values);
+
 /* ---- cross-validation (grid search) --------------------------------------
  * CrossValidationSelector (drivers/CrossValidationSelector.cpp:19-189) with
  * GridSearchCrossValidationDriver (drivers/GridSearchCrossValidationDriver.cpp:38-44,

@@ -201,6 +201,25 @@ def profile(data, prior, control, beta, indices, threshold=1.920729, include_pen
     return lo, up, ev
 
 
+def profile_likelihood(data, prior, control, beta, index, points, include_penalty=True):
+    """Profile log-likelihood of column `index` at each of `points` (evaluateProfileModel,
+    one clone per point started from the fit with the column held at 0)."""
+    L = lib()
+    L.orc_profile_likelihood.restype = C.c_int
+    L.orc_profile_likelihood.argtypes = [C.c_void_p, C.POINTER(_abi.FitArgs), C.POINTER(C.c_double), C.c_int,
+                                         C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_double)]
+    hold = _args(data, prior, control)
+    eng = OracleEngine(data)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    x = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.zeros(len(x))
+    rc = L.orc_profile_likelihood(eng.h, C.byref(hold.args), _d(b), int(index), _d(x), len(x),
+                                  int(bool(include_penalty)), _d(out))
+    if rc != 0:
+        raise RuntimeError(L.orc_last_error().decode())
+    return out
+
+
 def prior_delta(prior_type, variance, g, h, beta):
     """The prior's coordinate step (CovariatePrior.h getDelta) for gradient g, Hessian h."""
     L = lib()

@@ -973,6 +973,39 @@ int orc_profile(void* hv, const ccd_fit_args* a, const double* beta, const int* 
     }
 }
 
+/* CcdInterface::evaluateProfileModel (CcdInterface.cpp:527-607) with one clone per
+ * point: the column is held at 0 starting from beta (:570), each point's fit starts
+ * from that state (:572-577), values[i] = logLik (+ logPrior) of the fit. */
+int orc_profile_likelihood(void* hv, const ccd_fit_args* a, const double* beta, int index, const double* points,
+                           int n, int includePenalty, double* values) {
+    try {
+        auto* h = static_cast<orc_handle*>(hv);
+        const int J = h->data.J;
+        std::vector<double> start(beta, beta + J);
+        if (h->data.hasOffset) start[0] = 1.0;
+        auto evaluate = [&](std::vector<double>& b, double x) {
+            orc::CCD ccd(h->data, *h->ms);
+            orc::configure(ccd, a);
+            b[index] = x;
+            ccd.setBeta(b.data());
+            ccd.fixBeta[index] = 1;
+            ccd.update(*a);
+            ccd.fixBeta[index] = 0;
+            b.assign(ccd.beta.begin(), ccd.beta.end());
+            return ccd.logLik() + (includePenalty ? ccd.logPrior() : 0.0);
+        };
+        evaluate(start, 0.0);
+        for (int i = 0; i < n; ++i) {
+            std::vector<double> b = start;
+            values[i] = evaluate(b, points[i]);
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        g_orc_err = ex.what();
+        return 1;
+    }
+}
+
 double orc_prior_delta(int type, double variance, double g, double h, double beta) {
     orc::Prior p;
     p.type = type;

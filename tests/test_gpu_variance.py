@@ -183,3 +183,22 @@ def test_profile_refuses_regularized_column(E):
         assert lo[0] < b[0] < up[0]
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_profile_likelihood_grid_matches_oracle(E, model):
+    d = _cases.ALL[model]()
+    print(f"[case] profile likelihood grid {model}", flush=True)
+    prior = create_prior("normal", 2.0)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-10, max_iterations=5000)
+    col = d.J - 1
+    bo, _ = oracle.fit(d, prior, ctl)
+    pts = bo[col] + np.linspace(-1.0, 1.0, 9)
+    eng = E.Engine(d, 64)
+    try:
+        vg = eng.profile_likelihood(prior, ctl, bo, col, pts)
+    finally:
+        eng.close()
+    vo = oracle.profile_likelihood(d, prior, ctl, bo, col, pts)
+    assert np.allclose(vg, vo, rtol=1e-9, atol=1e-9 * np.max(np.abs(vo))), (vg, vo)
+    assert np.argmax(vg) == 4   # the mode sits at the centre of the grid

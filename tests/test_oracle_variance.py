@@ -202,3 +202,25 @@ def test_dobson_poisson_matches_published_glm():
     # without a prior the penalty changes nothing (tests/testthat/test-smallCLR.R:29-30)
     lo2, up2, _ = oracle.profile(d, prior, ctl, b, range(5), threshold=CHI95, include_penalty=False)
     assert np.array_equal(lo, lo2) and np.array_equal(up, up2)
+
+
+def test_profile_likelihood_grid_matches_an_independent_profile():
+    d = _bid_logistic()
+    X = d.to_dense()
+    prior = create_prior("none")
+    b, r = oracle.fit(d, prior, TIGHT)
+    pts = np.array([b[1] - 0.5, b[1] - 0.1, b[1], b[1] + 0.2, 0.0])
+    vals = oracle.profile_likelihood(d, prior, TIGHT, b, 1, pts)
+
+    def prof(x):   # intercept maximised with the slope held at x
+        def neg(z):
+            v = np.array([z[0], x])
+            p = 1 / (1 + np.exp(-(X @ v)))
+            return -ll_logistic(X, d.y, v), -np.array([X[:, 0] @ (d.y - p)])
+        z = minimize(neg, [b[0]], jac=True, method="BFGS", options={"gtol": 1e-10}).x
+        return -neg(z)[0]
+
+    expect = np.array([prof(x) for x in pts])
+    assert np.allclose(vals, expect, rtol=0, atol=1e-7), (vals, expect)
+    assert vals[2] == pytest.approx(r["log_likelihood"], abs=1e-9)   # at the mode: the maximum
+    assert np.all(vals <= r["log_likelihood"] + 1e-9)
