@@ -379,6 +379,30 @@ def confint(engine, prior, control, beta, parm=None, level=0.95, override_no_reg
     return np.column_stack([np.array(cols, dtype=np.float64), lo, up])
 
 
+def aconfint(engine, beta, parm=None, level=0.95):
+    """aconfint (R/ModelFit.R:848-869): asymptotic limits beta +- qnorm((1 - level) / 2) * SE,
+    the SEs from the inverse Fisher information of every covariate. Returns [m][3]:
+    column index, lower, upper."""
+    from scipy.stats import norm
+    if level < 0.01 or level > 0.99:
+        raise CyclopsError("level must be between 0 and 1")
+    allc = _covariates(engine.data, None)
+    ses = np.sqrt(np.diag(np.linalg.inv(engine.fisher_information(beta, allc))))
+    cols = _covariates(engine.data, parm)
+    se = np.array([ses[allc.index(c)] for c in cols])
+    a = (1.0 - level) / 2.0
+    b = np.asarray(beta, dtype=np.float64)[cols]
+    return np.column_stack([np.array(cols, dtype=np.float64), b + se * norm.ppf(a), b + se * norm.ppf(1.0 - a)])
+
+
+def predict(engine, beta):
+    """predict.cyclopsFit on the data of the fit (R/Predict.R:38-46, CcdInterface::predictModel):
+    getPredictiveEstimates at beta (LR: probability, PR: expected count per row)."""
+    engine.compute_xbeta(beta)
+    engine.remaining_stats()
+    return engine.pred_est()
+
+
 def get_cyclops_profile_log_likelihood(engine, prior, control, beta, parm, x, include_penalty=True):
     """getCyclopsProfileLogLikelihood (R/ModelFit.R:816-826): the profile log-likelihood
     of one covariate at each value of x."""

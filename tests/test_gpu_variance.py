@@ -202,3 +202,22 @@ def test_profile_likelihood_grid_matches_oracle(E, model):
     vo = oracle.profile_likelihood(d, prior, ctl, bo, col, pts)
     assert np.allclose(vg, vo, rtol=1e-9, atol=1e-9 * np.max(np.abs(vo))), (vg, vo)
     assert np.argmax(vg) == 4   # the mode sits at the centre of the grid
+
+
+def test_aconfint_and_predict_dobson(E):
+    d, g = _dobson()
+    prior = create_prior("none")
+    ctl = create_control(tolerance=1e-8)
+    eng = E.Engine(d, 64)
+    try:
+        b, _ = eng.fit(prior, ctl, mode="resident")
+        ci = E.aconfint(eng, b, level=0.95)
+        mu = E.predict(eng, b)
+    finally:
+        eng.close()
+    se = np.array(g["std_error"])
+    assert np.allclose(ci[:, 1], b - 1.959963984540054 * se, atol=2e-4)
+    assert np.allclose(ci[:, 2], b + 1.959963984540054 * se, atol=2e-4)
+    # fitted counts of the additive log-linear model: row total x column total / N
+    outcome = np.tile([0, 1, 2], 3)
+    assert np.allclose(mu, 50.0 * np.array([63.0, 40.0, 47.0])[outcome] / 150.0, rtol=1e-6)
