@@ -221,3 +221,28 @@ def test_aconfint_and_predict_dobson(E):
     # fitted counts of the additive log-linear model: row total x column total / N
     outcome = np.tile([0, 1, 2], 3)
     assert np.allclose(mu, 50.0 * np.array([63.0, 40.0, 47.0])[outcome] / 150.0, rtol=1e-6)
+
+
+def test_variance_entry_points_refuse_bad_arguments(E):
+    d = _cases.poisson()   # column 0 is the log-time offset
+    prior = create_prior("none")
+    ctl = create_control(tolerance=1e-8)
+    eng = E.Engine(d, 64)
+    try:
+        b, _ = eng.fit(prior, ctl, mode="resident")
+        with pytest.raises(E.CyclopsError, match="out of range"):
+            eng.fisher_information(b, [0, d.J])
+        assert eng.fisher_information(b, []).shape == (0, 0)
+        with pytest.raises(E.CyclopsError, match="fixed"):
+            eng.profile(prior, ctl, b, [0], CHI95)          # the offset is fixed at 1
+        with pytest.raises(E.CyclopsError, match="not found"):
+            eng.profile(prior, ctl, b, [d.J], CHI95)
+        with pytest.raises(E.CyclopsError, match="threshold"):
+            eng.profile(prior, ctl, b, [1], 0.0)
+        with pytest.raises(E.CyclopsError, match="fixed"):
+            eng.profile_likelihood(prior, ctl, b, 0, [0.5])
+        # a failed call leaves the engine usable
+        lo, up, _ = eng.profile(prior, ctl, b, [1], CHI95)
+        assert lo[0] < b[1] < up[0]
+    finally:
+        eng.close()
