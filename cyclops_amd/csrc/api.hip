@@ -510,6 +510,22 @@ int ccd_fit_batch(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args
     });
 }
 
+int ccd_fit_multi(const ccd_handle* handles, int32_t n, const ccd_fit_args* args, double* const* betas,
+                  ccd_fit_result* results) {
+    return guard([&] {
+        if (!handles || !args || !betas || !results) throw std::invalid_argument("null argument");
+        if (n < 1) throw std::invalid_argument("n must be >= 1");
+        std::vector<ccd::EngineBase*> e(n);
+        for (int i = 0; i < n; ++i) {
+            if (!betas[i]) throw std::invalid_argument("null argument");
+            e[i] = &E(handles[i]);
+            ccd::checkArgs(*e[i], args[i]);
+            std::memset(&results[i], 0, sizeof(results[i]));
+        }
+        ccd::fitMulti(e.data(), n, args, betas, results);
+    });
+}
+
 int ccd_fisher_information(ccd_handle h, const double* beta, const int32_t* indices, int32_t m, double* out) {
     return guard([&] {
         if (!beta || (m > 0 && (!indices || !out))) throw std::invalid_argument("null argument");

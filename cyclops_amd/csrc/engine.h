@@ -113,6 +113,10 @@ ResidentNeeds residentNeeds(const ccd_data_desc& d, int device);
 // (CyclicCoordinateDescent.cpp:586-615, 909-1216). This is the plain per-fit path.
 void fitPlain(EngineBase& eng, const ccd_fit_args& a, double* beta, ccd_fit_result& r);
 
+// Single fits of several datasets (engines of one model and precision on one
+// device) as shared persistent launches (resident.inl).
+void fitMulti(EngineBase* const* engs, int n, const ccd_fit_args* args, double* const* betas, ccd_fit_result* res);
+
 // Profile-likelihood limits (CcdInterface::profileModel, CcdInterface.cpp:262-434): all
 // bracket and root searches advance together, one batched fit per round (profile.hip).
 void profileModel(EngineBase& eng, const ccd_fit_args& a, const double* beta, const int32_t* idx, int m,

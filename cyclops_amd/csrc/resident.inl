@@ -299,13 +299,32 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
     return out;
 }
 
-// Run fits [0, B) (host arrays indexed by fit) as persistent launches until all
-// have finished; fills betas/flags/iters/ll. kwHost: row weights of a single fit.
+// A prepared persistent run: the fits' device state and one parameter block per team.
 template <typename Real>
-static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
-                        const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
-                        int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps,
-                        int& nLaunches, Layout L, const int32_t* fixJ = nullptr) {
+struct ResRun {
+    EngineT<Real>* E = nullptr;
+    ResWork<Real>* W = nullptr;
+    int B = 0, J = 0, BP = 0, nTeams = 0, G = 0, m = 0, rateKHz = 100000;
+    std::vector<res::Params<Real>> teamP, blockP;
+    // the kernel reads workgroup i's block: its team's parameters with w = i mod G
+    void fill() {
+        blockP.resize((size_t)nTeams * G);
+        for (int t = 0; t < nTeams; ++t)
+            for (int w = 0; w < G; ++w) {
+                blockP[(size_t)t * G + w] = teamP[t];
+                blockP[(size_t)t * G + w].w = w;
+            }
+    }
+};
+
+// Set up fits [0, B) (host arrays indexed by fit) of engine E for persistent
+// launches: structure, per-batch buffers, inputs, xb, parameter blocks.
+// kwHost: row weights of a single fit.
+template <typename Real>
+static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var,
+                                    const int32_t* held, const int32_t* rowFold, int F, const double* kwHost,
+                                    const double* initBound, const double* betas, Layout L,
+                                    const int32_t* fixJ = nullptr) {
     auto& D = *E.D;
     hipStream_t s = E.s;
     const int64_t K = D.K;
@@ -565,18 +584,61 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
         Q.slots = W.slots + (size_t)t * 2 * G * BP * res::SLOTW;
         Q.bcast = W.bcast + (size_t)t * 2 * BP * 4;
     }
-    // the kernel reads workgroup i's block: its team's parameters with w = i mod G
-    std::vector<res::Params<Real>> blockP((size_t)nTeams * G);
-    auto fillBlocks = [&]() {
-        for (int t = 0; t < nTeams; ++t)
-            for (int w = 0; w < G; ++w) {
-                blockP[(size_t)t * G + w] = teamP[t];
-                blockP[(size_t)t * G + w].w = w;
-            }
-        h2d(W.teams, blockP.data(), blockP.size(), s);
+    ResRun<Real> R;
+    R.E = &E;
+    R.W = &W;
+    R.B = B;
+    R.J = J;
+    R.BP = BP;
+    R.nTeams = nTeams;
+    R.G = G;
+    R.m = m;
+    R.rateKHz = rateKHz;
+    R.teamP = std::move(teamP);
+    return R;
+}
+
+// Launch prepared runs until all their fits have finished. One run launches its
+// own grid; several runs (same model, precision and fit padding) share ONE grid,
+// each run's teams keeping their own sizes, exchanges and state, and one error and
+// completion flag. Each launch runs up to `budget` cycles; state stays in HBM.
+template <typename Real>
+static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs, int& nLaunches) {
+    ResRun<Real>& R0 = *runs[0];
+    auto& W0 = *R0.W;
+    EngineT<Real>& E0 = *R0.E;
+    hipStream_t s = E0.s;
+    const bool fused = runs.size() > 1;
+    int total = 0, fitsAll = 0;
+    bool exchanges = false;
+    for (auto* R : runs) {
+        if (R->m != R0.m || R->BP != R0.BP) throw std::logic_error("fused runs need one kernel instance");
+        total += R->nTeams * R->G;
+        fitsAll += R->B;
+        exchanges = exchanges || R->G > 1;
+    }
+    res::Params<Real>* dTeams = W0.teams;
+    double fb = 0.0;
+    if (fused) {
+        for (auto* R : runs) CCD_HIP(hipStreamSynchronize(R->E->s));   // every run's set-up is on the device
+        for (auto* R : runs)
+            for (auto& Q : R->teamP) { Q.err = W0.err; Q.allDoneOut = W0.allDone; }
+        dTeams = dalloc<res::Params<Real>>(total, fb);
+    }
+    std::vector<res::Params<Real>> all;
+    auto upload = [&]() {
+        if (!fused) {
+            R0.fill();
+            h2d(W0.teams, R0.blockP.data(), R0.blockP.size(), s);
+            return;
+        }
+        all.clear();
+        for (auto* R : runs) {
+            R->fill();
+            all.insert(all.end(), R->blockP.begin(), R->blockP.end());
+        }
+        h2d(dTeams, all.data(), all.size(), s);
     };
-    fillBlocks();
-    // ---- launches: each runs up to `budget` cycles, state stays in HBM -----------
     // Cycles per launch: CCD_CYCLES_PER_LAUNCH, else adapted so a launch lasts about
     // 4 s (the host sees progress and errors at launch boundaries): 2 cycles first,
     // then as many as the measured cycle time allows, at most 32.
@@ -585,49 +647,67 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) { budget = std::max(1, std::atoi(e)); fixedBudget = true; }
     const bool progress = std::getenv("CCD_PROGRESS") != nullptr;
     const auto tWall = std::chrono::steady_clock::now();
-    CCD_HIP(hipEventRecord(E.ev0, s));
+    CCD_HIP(hipEventRecord(E0.ev0, s));
     int done = 0, launches = 0, cyclesRun = 0, lastBudget = -1;
     const int one = 1;
     while (!done) {
-        CCD_HIP(hipMemsetAsync(W.epoch, 0, (size_t)nTeams * G * sizeof(unsigned), s));
-        CCD_HIP(hipMemsetAsync(W.err, 0, sizeof(int), s));
-        h2d(W.allDone, &one, 1, s);
+        for (auto* R : runs)
+            CCD_HIP(hipMemsetAsync(R->W->epoch, 0, (size_t)R->nTeams * R->G * sizeof(unsigned), s));
+        CCD_HIP(hipMemsetAsync(W0.err, 0, sizeof(int), s));
+        h2d(W0.allDone, &one, 1, s);
         if (launches <= 1 || budget != lastBudget) {   // the first launch starts the fits, later ones resume
-            for (auto& Q : teamP) { Q.resume = launches > 0 ? 1 : 0; Q.cycleBudget = budget; }
-            fillBlocks();
+            for (auto* R : runs)
+                for (auto& Q : R->teamP) { Q.resume = launches > 0 ? 1 : 0; Q.cycleBudget = budget; }
+            upload();
             lastBudget = budget;
         }
         const auto tL = std::chrono::steady_clock::now();
-        switch (m) {
-        case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(BP, W.teams, nTeams, G, s); break;
-        case M_POISSON: launchByBP<Real, M_POISSON>(BP, W.teams, nTeams, G, s); break;
-        case M_CLR: launchByBP<Real, M_CLR>(BP, W.teams, nTeams, G, s); break;
-        default: launchByBP<Real, M_COX>(BP, W.teams, nTeams, G, s); break;
+        // teams that exchange need every workgroup of the grid resident at once
+        const int nT = fused ? (exchanges ? 1 : total) : R0.nTeams;
+        const int T = fused ? (exchanges ? total : 1) : R0.G;
+        switch (R0.m) {
+        case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(R0.BP, dTeams, nT, T, s); break;
+        case M_POISSON: launchByBP<Real, M_POISSON>(R0.BP, dTeams, nT, T, s); break;
+        case M_CLR: launchByBP<Real, M_CLR>(R0.BP, dTeams, nT, T, s); break;
+        default: launchByBP<Real, M_COX>(R0.BP, dTeams, nT, T, s); break;
         }
         CCD_HIP(hipGetLastError());
         int hs[2] = {0, 0};
-        CCD_HIP(hipMemcpyAsync(&hs[0], W.err, sizeof(int), hipMemcpyDeviceToHost, s));
-        CCD_HIP(hipMemcpyAsync(&hs[1], W.allDone, sizeof(int), hipMemcpyDeviceToHost, s));
+        CCD_HIP(hipMemcpyAsync(&hs[0], W0.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        CCD_HIP(hipMemcpyAsync(&hs[1], W0.allDone, sizeof(int), hipMemcpyDeviceToHost, s));
         CCD_HIP(hipStreamSynchronize(s));
-        if (hs[0]) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
+        if (hs[0]) {
+            if (fused) dfree(dTeams);
+            throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
+        }
         done = hs[1];
         ++launches;
         cyclesRun += budget;
         if (progress)
-            std::fprintf(stderr, "[ccd] launch %d (%d teams x %d workgroups, %d fits): %d cycles, %s (%.1fs)\n",
-                         launches, nTeams, G, B, cyclesRun, done ? "all fits finished" : "fits still running",
+            std::fprintf(stderr, "[ccd] launch %d (%d runs, %d workgroups, %d fits): %d cycles, %s (%.1fs)\n",
+                         launches, (int)runs.size(), total, fitsAll, cyclesRun,
+                         done ? "all fits finished" : "fits still running",
                          std::chrono::duration<double>(std::chrono::steady_clock::now() - tWall).count());
         if (!fixedBudget && !done) {
             const double perCycle = std::chrono::duration<double>(std::chrono::steady_clock::now() - tL).count() / budget;
             budget = (int)std::max(1.0, std::min(32.0, std::floor(4.0 / std::max(perCycle, 1e-6))));
         }
     }
-    CCD_HIP(hipEventRecord(E.ev1, s));
+    CCD_HIP(hipEventRecord(E0.ev1, s));
     CCD_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
-    CCD_HIP(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+    CCD_HIP(hipEventElapsedTime(&ms, E0.ev0, E0.ev1));
     deviceMs = ms;
     nLaunches = launches;
+    if (fused) dfree(dTeams);
+}
+
+// Results of a finished run.
+template <typename Real>
+static void gatherRun(ResRun<Real>& R, double* betas, int* flags, int* iters, double* lls, double& algoBytes,
+                      double& steps) {
+    auto& W = *R.W;
+    const int B = R.B, J = R.J;
     unsigned long long st = 0;
     CCD_HIP(hipMemcpy(&st, W.steps, sizeof(st), hipMemcpyDeviceToHost));
     steps = (double)st;
@@ -639,8 +719,22 @@ static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const do
     CCD_HIP(hipMemcpy(st8, W.stats, sizeof(st8), hipMemcpyDeviceToHost));
     algoBytes = st8[0];
     if (std::getenv("CCD_PHASES"))
-        std::fprintf(stderr, "[ccd] teams=%d T=%d B=%d BP=%d steps=%llu  ms (summed over teams): column pass %.1f  exchange %.1f  update %.1f  objective %.1f  (kernel %.1f)\n",
-                     nTeams, G, B, BP, st, st8[1] / rateKHz, st8[2] / rateKHz, st8[3] / rateKHz, st8[4] / rateKHz, (double)ms);
+        std::fprintf(stderr, "[ccd] teams=%d T=%d B=%d BP=%d steps=%llu  ms (summed over teams): column pass %.1f  exchange %.1f  update %.1f  objective %.1f\n",
+                     R.nTeams, R.G, B, R.BP, st, st8[1] / R.rateKHz, st8[2] / R.rateKHz, st8[3] / R.rateKHz,
+                     st8[4] / R.rateKHz);
+}
+
+// Run fits [0, B) of engine E as persistent launches until all have finished;
+// fills betas/flags/iters/ll.
+template <typename Real>
+static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
+                        const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
+                        int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps,
+                        int& nLaunches, Layout L, const int32_t* fixJ = nullptr) {
+    ResRun<Real> R = prepareResident<Real>(E, a, B, var, held, rowFold, F, kwHost, initBound, betas, L, fixJ);
+    std::vector<ResRun<Real>*> one{&R};
+    launchRuns<Real>(one, deviceMs, nLaunches);
+    gatherRun<Real>(R, betas, flags, iters, lls, algoBytes, steps);
 }
 
 template <typename Real>
@@ -822,6 +916,113 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
         r[f].log_likelihood = lls[f];
         r[f].log_prior = lp;
         r[f].seconds = secs;
+    }
+}
+
+// Single fits of several datasets as one persistent grid (CyclicCoordinateDescent::
+// update, CyclicCoordinateDescent.cpp:586-615, for every engine at once). Each fit
+// keeps the team shape it would have alone, so its result is the same as its own
+// resident fit; fits are packed into grids whose workgroups can all be resident.
+template <typename Real>
+static void fitMultiT(const std::vector<EngineT<Real>*>& engs, const ccd_fit_args* args, double* const* betas,
+                      ccd_fit_result* res) {
+    const int n = (int)engs.size();
+    const auto t0 = std::chrono::steady_clock::now();
+    const int device = engs[0]->D->device;
+    const int model = engs[0]->D->model;
+    const bool grouped = model == M_CLR || model == M_COX;
+    for (int i = 0; i < n; ++i) {
+        if (engs[i]->D->device != device || engs[i]->D->model != model)
+            throw std::invalid_argument("fit_multi needs engines of one model type on one device");
+        for (int k = 0; k < i; ++k)
+            if (engs[k] == engs[i]) throw std::invalid_argument("fit_multi needs distinct engines");
+    }
+    CCD_HIP(hipSetDevice(device));
+    int cus = 0;
+    CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int capacity = std::max(1, occupancyOf<Real>(model, 1) * cus);
+    std::vector<std::vector<double>> b(n);
+    std::vector<double> bound(n), var(n), lls(n, 0.0);
+    std::vector<int> count(n, 0), flags(n, 0), its(n, 0);
+    std::vector<Layout> lay(n);
+    for (int i = 0; i < n; ++i) {
+        const int J = engs[i]->D->J;
+        b[i].assign(betas[i], betas[i] + J);
+        if (engs[i]->D->hasOffset) b[i][0] = 1.0;
+        bound[i] = args[i].initial_bound;
+        var[i] = args[i].variance;
+        lay[i] = chooseLayout(engs[i]->D->K, 1, device, grouped);   // the shape of its own resident fit
+        if (lay[i].T > capacity) throw std::invalid_argument("a fit needs more workgroups than the device holds");
+    }
+    std::vector<int> pending(n);
+    std::iota(pending.begin(), pending.end(), 0);
+    while (!pending.empty()) {
+        std::vector<int> retry;
+        for (size_t g0 = 0; g0 < pending.size();) {
+            // next grid: as many pending fits as fit co-resident
+            std::vector<int> grp;
+            int used = 0;
+            while (g0 < pending.size() && (grp.empty() || used + lay[pending[g0]].T <= capacity)) {
+                used += lay[pending[g0]].T;
+                grp.push_back(pending[g0++]);
+            }
+            std::vector<ResRun<Real>> runs;
+            runs.reserve(grp.size());
+            for (int i : grp)
+                runs.push_back(prepareResident<Real>(*engs[i], args[i], 1, &var[i], nullptr, nullptr, 0,
+                                                     args[i].weights, &bound[i], b[i].data(), lay[i]));
+            std::vector<ResRun<Real>*> ptrs;
+            for (auto& R : runs) ptrs.push_back(&R);
+            double ms = 0.0;
+            int nl = 0;
+            launchRuns<Real>(ptrs, ms, nl);
+            for (size_t q = 0; q < grp.size(); ++q) {
+                const int i = grp[q];
+                double bytes = 0.0, steps = 0.0;
+                gatherRun<Real>(runs[q], b[i].data(), &flags[i], &its[i], &lls[i], bytes, steps);
+                ++count[i];
+                if (flags[i] == CCD_ILLCONDITIONED && count[i] < args[i].max_bound_count) {   // update() retry
+                    bound[i] /= 10.0;
+                    for (int j = engs[i]->D->hasOffset ? 1 : 0; j < engs[i]->D->J; ++j) b[i][j] = 0.0;
+                    retry.push_back(i);
+                }
+            }
+        }
+        pending.swap(retry);
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < n; ++i) {
+        const int J = engs[i]->D->J;
+        std::copy(b[i].begin(), b[i].end(), betas[i]);
+        double lp = 0.0;
+        for (int j = 0; j < J; ++j)
+            lp += priorLogDensity((args[i].exclude && args[i].exclude[j]) ? CCD_PRIOR_NONE : args[i].prior_type,
+                                  args[i].variance, b[i][j]);
+        res[i].return_flag = flags[i];
+        res[i].iterations = its[i];
+        res[i].find_mode_calls = count[i];
+        res[i].log_likelihood = lls[i];
+        res[i].log_prior = lp;
+        res[i].seconds = secs;
+    }
+}
+
+void fitMulti(EngineBase* const* engs, int n, const ccd_fit_args* args, double* const* betas, ccd_fit_result* res) {
+    if (n < 1) throw std::invalid_argument("n must be >= 1");
+    if (engs[0]->precision() == CCD_FP32) {
+        std::vector<EngineT<float>*> e(n);
+        for (int i = 0; i < n; ++i) {
+            e[i] = dynamic_cast<EngineT<float>*>(engs[i]);
+            if (!e[i]) throw std::invalid_argument("fit_multi needs engines of one precision");
+        }
+        fitMultiT<float>(e, args, betas, res);
+    } else {
+        std::vector<EngineT<double>*> e(n);
+        for (int i = 0; i < n; ++i) {
+            e[i] = dynamic_cast<EngineT<double>*>(engs[i]);
+            if (!e[i]) throw std::invalid_argument("fit_multi needs engines of one precision");
+        }
+        fitMultiT<double>(e, args, betas, res);
     }
 }
 

@@ -68,6 +68,8 @@ def lib():
                                          C.POINTER(_abi.FitResult), C.POINTER(_abi.CvResult)]
         L.ccd_cv_assign_folds.argtypes = [C.c_int64, C.POINTER(i32), i32, i32, C.c_int64, C.POINTER(i32)]
         L.ccd_fisher_information.argtypes = [vp, dp, C.POINTER(i32), i32, dp]
+        L.ccd_fit_multi.argtypes = [C.POINTER(vp), i32, C.POINTER(_abi.FitArgs), C.POINTER(dp),
+                                    C.POINTER(_abi.FitResult)]
         L.ccd_profile.argtypes = [vp, C.POINTER(_abi.FitArgs), dp, C.POINTER(i32), i32, C.c_double, C.c_int,
                                   C.c_int, dp, dp, C.POINTER(i32)]
         L.ccd_profile_likelihood.argtypes = [vp, C.POINTER(_abi.FitArgs), dp, i32, dp, i32, C.c_int, dp]
@@ -331,6 +333,24 @@ def fit_cyclops_model(data, prior=None, control=None, weights=None, fixed=None, 
         return eng.fit(prior, control, weights=weights, fixed=fixed, beta0=beta0, mode=mode)
     finally:
         eng.close()
+
+
+def fit_multi(engines, priors, controls, weights=None, betas0=None):
+    """One fit per engine (each its own dataset), all as shared persistent launches
+    (ccd_fit_multi). Returns [(beta, result dict)] in engine order; each equals the
+    engine's own resident fit."""
+    n = len(engines)
+    weights = weights or [None] * n
+    holds = [FitArgsHolder(e.data.J, p, c, weights=w, intercept_index=intercept_index(e.data))
+             for e, p, c, w in zip(engines, priors, controls, weights)]
+    args = (_abi.FitArgs * n)(*[h.args for h in holds])
+    betas = [np.zeros(e.data.J) if betas0 is None or betas0[i] is None else np.array(betas0[i], dtype=np.float64)
+             for i, e in enumerate(engines)]
+    ptrs = (C.POINTER(C.c_double) * n)(*[_d(b) for b in betas])
+    hs = (C.c_void_p * n)(*[e.h for e in engines])
+    res = (_abi.FitResult * n)()
+    check(lib().ccd_fit_multi(hs, n, args, ptrs, res))
+    return [(b, r.as_dict()) for b, r in zip(betas, res)]
 
 
 def _covariates(data, covariates):

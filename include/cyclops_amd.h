@@ -215,6 +215,19 @@ typedef struct ccd_batch_args {
 int ccd_fit_batch(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args* batch,
                   double* betas, ccd_fit_result* results);
 
+/* ---- fits of several datasets in one launch ---------------------------------
+ * One fit per handle (its own data, args[i] and starting betas[i] [J_i], overwritten
+ * with the estimate), as CyclicCoordinateDescent::update (CyclicCoordinateDescent.cpp:
+ * 586-615) on each dataset, run as shared persistent launches: every fit keeps the
+ * team shape of its own resident fit (so the results equal ccd_fit(..., RESIDENT)),
+ * and the fits are packed into grids whose workgroups are all resident together.
+ * The handles must be distinct engines of one model type and precision on one device;
+ * ccd_fit_args.weights may differ per fit. This is the batching of uneven datasets
+ * into one fused launch; a caller would use it in place of fitting the datasets one
+ * after another. */
+int ccd_fit_multi(const ccd_handle* handles, int32_t n, const ccd_fit_args* args, double* const* betas,
+                  ccd_fit_result* results);
+
 /* ---- variance and profile likelihood ---------------------------------------
  * Fisher information of the columns indices[0..m) at beta:
  * CyclicCoordinateDescent::computeFisherInformation (CyclicCoordinateDescent.cpp:1288-1300)
