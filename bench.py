@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=3, help="CCD cycles of the CPU baseline sample (about 10 s)")
-    ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic", "variance"],
+    ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic", "variance", "multi"],
                     default="poisson",
                     help="poisson: the batched-fit headline line (default); serving: the request-mix server; "
                          "lr / cox / clr-cv: the other benchmark workloads")
@@ -71,6 +71,8 @@ def main():
         return run_traffic(args)
     if args.workload == "variance":
         return run_variance(args)
+    if args.workload == "multi":
+        return run_multi(args)
     if args.workload != "poisson":
         return run_other(args)
 
@@ -556,6 +558,57 @@ def run_variance(args):
         "fisher_information": {"columns": m, "ms": t_fis * 1e3, "tflops_fp64": flops / t_fis / 1e12,
                                "peak_tflops_fp64": 78.6, "symmetric": bool(np.array_equal(F, F.T))},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_multi(args):
+    """Uneven datasets in one persistent launch: the logistic requests of the serving
+    generator (rows log-uniform in [1e4, 1e5], columns in [1e3, 1e4], FP32 bulk), one fit
+    each, run by ccd_fit_multi against the same fits one after another on the resident
+    path. Both give the same coefficients."""
+    from cyclops_amd import _abi
+    from cyclops_amd import data as D
+    from cyclops_amd import engine as E
+    from cyclops_amd.control import create_control, create_prior
+
+    if E.device_count() < 1:
+        raise SystemExit("no HIP device")
+    reqs, r = [], 0
+    while len(reqs) < 24:
+        q = D.make_request(r, max_rows=100_000, max_cols=10_000)
+        r += 1
+        if q["model"] == "lr":
+            reqs.append(q)
+    log(f"{len(reqs)} logistic datasets, rows {min(q['data'].K for q in reqs)}..{max(q['data'].K for q in reqs)}")
+    ctl = create_control(convergence_type="mittal", tolerance=1e-6)
+    priors = [create_prior(q["prior"], q["variance"]) for q in reqs]
+    engs = [E.Engine(q["data"], _abi.FP32) for q in reqs]
+    try:
+        E.fit_multi(engs[:2], priors[:2], [ctl] * 2)   # warm-up: module load, workspaces
+        t0 = time.perf_counter()
+        alone = [e.fit(p, ctl, mode="resident") for e, p in zip(engs, priors)]
+        t_alone = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        together = E.fit_multi(engs, priors, [ctl] * len(engs))
+        t_multi = time.perf_counter() - t0
+    finally:
+        for e in engs:
+            e.close()
+    same = all(np.array_equal(a[0], b[0]) for a, b in zip(alone, together))
+    n = len(reqs)
+    log(f"one launch: {t_multi:.2f}s; one after another: {t_alone:.2f}s; identical coefficients: {same}")
+    out = {
+        "metric": "fits/sec of uneven datasets in one persistent launch vs one after another",
+        "value": n / t_multi, "unit": "fits/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": t_multi * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 bulk (f64 prior step, reductions and convergence test)",
+        "data": "synthetic (one seeded generator per dataset, no dataset files)",
+        "config": {"workload": "uneven_logistic_datasets_24_rows1e4-1e5_cols1e3-1e4", "fits": n,
+                   "rows": [q["data"].K for q in reqs], "cols": [q["data"].J for q in reqs]},
+        "one_after_another": {"seconds": t_alone, "fits_per_sec": n / t_alone},
+        "speedup": t_alone / t_multi,
+        "identical_coefficients": same,
     }
     print(json.dumps(out), flush=True)
 
