@@ -224,3 +224,25 @@ def test_profile_likelihood_grid_matches_an_independent_profile():
     assert np.allclose(vals, expect, rtol=0, atol=1e-7), (vals, expect)
     assert vals[2] == pytest.approx(r["log_likelihood"], abs=1e-9)   # at the mode: the maximum
     assert np.all(vals <= r["log_likelihood"] + 1e-9)
+
+
+def test_warm_start_cv_chain_reaches_the_cold_start_optima():
+    """The chain only changes where each fit starts: at a tight tolerance every fold x
+    grid fit reaches the same penalised optimum as from zero."""
+    from cyclops_amd import _abi
+    d = _cases.logistic()
+    prior = create_prior("laplace", 1.0)
+    out = {}
+    for reset in (True, False):
+        ctl = create_control(convergence_type="mittal", tolerance=1e-12, max_iterations=100000, fold=4,
+                             grid_size=5, lower_limit=0.01, upper_limit=5.0, seed=123, reset_coefficients=reset)
+        out[reset] = oracle.cross_validate(d, prior, ctl, _abi.SELECT_BY_ROW)
+    assert np.allclose(out[True]["pred"], out[False]["pred"], rtol=0, atol=1e-7)
+    assert out[True]["best_variance"] == out[False]["best_variance"]
+    # and at the default tolerance the chain is not the cold start (it starts elsewhere)
+    loose = {}
+    for reset in (True, False):
+        ctl = create_control(convergence_type="mittal", tolerance=1e-8, fold=4, grid_size=5, lower_limit=0.01,
+                             upper_limit=5.0, seed=123, reset_coefficients=reset)
+        loose[reset] = oracle.cross_validate(d, prior, ctl, _abi.SELECT_BY_ROW)
+    assert not np.array_equal(loose[True]["pred"], loose[False]["pred"])
