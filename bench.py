@@ -518,8 +518,7 @@ def run_variance(args):
     F = eng.fisher_information(b, allc)
     t_fis = time.perf_counter() - t0
     m = d.J
-    tiles = (m + 31) // 32
-    flops = 2.0 * d.K * (tiles * (tiles + 1) // 2) * 32 * 32   # FMAs of the upper tiles, x2
+    flops = float(d.K) * m * (m + 1)   # one FMA per row for each entry of the upper triangle
     log(f"fisher information {m} x {m}: {t_fis * 1e3:.1f} ms ({flops / t_fis / 1e12:.2f} TFLOP/s FP64)")
     # profile limits: all columns together, then one column at a time
     t0 = time.perf_counter()

@@ -16,13 +16,13 @@
 // On the device both terms are weighted Gram matrices of a dense row-major block:
 // the selected columns are scattered into Z[rows][m] (a row chunk at a time, so Z
 // stays within ZBYTES), the grouped models fold Z into C[N][m], and kGram forms the
-// upper 32 x 32 tiles of Z^T diag(w) Z in FP64 over row splits whose partial sums
+// upper 64 x 64 tiles of Z^T diag(w) Z in FP64 over row splits whose partial sums
 // kGramReduce adds in a fixed order: the result is the same on every run.
 
 namespace ccd {
 namespace fis {
 
-constexpr int GT = 32;                       // Gram tile edge
+constexpr int GT = 64;                       // Gram tile edge
 constexpr int GR = 32;                       // rows staged in LDS per step
 constexpr size_t ZBYTES = size_t(256) << 20; // row-chunk buffer
 
@@ -101,9 +101,10 @@ __global__ __launch_bounds__(BLOCK) void kGroupSums(int64_t n0, int64_t n1, int 
     }
 }
 
-// Partial Gram of the tile pair blockIdx.x (ti <= tj over nt tiles) on row split
-// blockIdx.y: part[y][i][j] = sum_r w_r Z[r][i] Z[r][j] for i in tile ti, j in tile tj.
-// 256 threads: thread (tx = column in tj, ty) owns rows ty, ty+8, ty+16, ty+24 of ti.
+// Partial Gram of the tile pair blockIdx.x (ti <= tj over nt tiles of GT columns) on
+// row split blockIdx.y: part[y][i][j] = sum_r w_r Z[r][i] Z[r][j] for i in tile ti,
+// j in tile tj. 256 threads as 16 x 16; thread (tx, ty) owns the 4 x 4 outputs
+// i = ty + 16a, j = tx + 16b, so each staged row costs 8 LDS reads for 16 FMAs.
 __global__ __launch_bounds__(256) void kGram(int64_t R, int m, int nt, const double* __restrict__ Z,
                                              const double* __restrict__ w, int64_t rowsPerSplit,
                                              double* __restrict__ part) {
@@ -113,35 +114,49 @@ __global__ __launch_bounds__(256) void kGram(int64_t R, int m, int nt, const dou
     int p = blockIdx.x, ti = 0;
     while (p >= nt - ti) { p -= nt - ti; ++ti; }
     const int tj = ti + p;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int64_t rs0 = (int64_t)blockIdx.y * rowsPerSplit;
     const int64_t rs1 = min(R, rs0 + rowsPerSplit);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int ci = ti * GT + tx, cj = tj * GT + tx;
-    for (int64_t rb = rs0; rb < rs1; rb += GR) {
-        // stage GR rows of both column tiles; the weight goes on the ti side
+    double acc[4][4];
 #pragma unroll
-        for (int q = 0; q < GR / 8; ++q) {
-            const int rr = ty + 8 * q;
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    // staging: thread t loads column c = t % GT of rows t / GT + 4q (GR x GT per matrix)
+    const int lc = threadIdx.x % GT, lr = threadIdx.x / GT;
+    const int ci = ti * GT + lc, cj = tj * GT + lc;
+    for (int64_t rb = rs0; rb < rs1; rb += GR) {
+#pragma unroll
+        for (int q = 0; q < GR / (256 / GT); ++q) {
+            const int rr = lr + q * (256 / GT);
             const int64_t r = rb + rr;
             const bool in = r < rs1;
-            zi[rr][tx] = (in && ci < m) ? w[r] * Z[r * m + ci] : 0.0;
-            zj[rr][tx] = (in && cj < m) ? Z[r * m + cj] : 0.0;
+            zi[rr][lc] = (in && ci < m) ? w[r] * Z[r * m + ci] : 0.0;
+            zj[rr][lc] = (in && cj < m) ? Z[r * m + cj] : 0.0;
         }
         __syncthreads();
-#pragma unroll 8
+#pragma unroll 4
         for (int rr = 0; rr < GR; ++rr) {
-            const double b = zj[rr][tx];
+            double u[4], v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] += zi[rr][ty + 8 * q] * b;
+            for (int a = 0; a < 4; ++a) u[a] = zi[rr][ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) v[b] = zj[rr][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] += u[a] * v[b];
         }
         __syncthreads();
     }
-    if (cj < m) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = ti * GT + ty + 8 * q;
-            if (i < m) part[((size_t)blockIdx.y * m + i) * m + cj] = acc[q];
+    for (int a = 0; a < 4; ++a) {
+        const int i = ti * GT + ty + 16 * a;
+        if (i >= m) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = tj * GT + tx + 16 * b;
+            if (j < m) part[((size_t)blockIdx.y * m + i) * m + j] = acc[a][b];
         }
     }
 }
