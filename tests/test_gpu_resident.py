@@ -215,3 +215,30 @@ def test_fold_assignment_matches_oracle(E):
         a = E.assign_folds(K, pid, 10, sel, 42)
         b = oracle.assign_folds(K, pid, 10, sel, 42)
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("conv", ["gradient", "mittal"])
+@pytest.mark.parametrize("mode", ["plain", "resident"])
+def test_separable_logistic_follows_the_oracle(E, mode, conv):
+    """A covariate that is non-zero only on successes (tests/testthat/test-largeBernoulli.R:24-46):
+    without a prior its coefficient runs off along a flat direction of the likelihood. There
+    the Hessian p(1 - p) of rows with |x beta| >> 1 cancels, so a one-ulp difference between
+    the device's and the host's exp moves the steps; the stopping cycle then depends on the
+    implementation (as it does between the reference's own builds). What must agree is the
+    outcome: the return flag, the log-likelihood and the run-off coefficient."""
+    from cyclops_amd import data as D
+    rng = np.random.default_rng(666)
+    K = 500
+    X = (rng.random((K, 5)) < 0.3).astype(float)
+    p = 1 / (1 + np.exp(-(X @ np.array([0.5, -0.3, 0.8, 0.2, -0.6]) - 0.5)))
+    y = (rng.random(K) < p).astype(float)
+    d = D.from_dense(np.column_stack([X, X[:, 2] * y]), y, "lr", add_intercept=True)
+    print(f"[case] separable logistic {mode} {conv}", flush=True)
+    ctl = create_control(convergence_type=conv)
+    bo, ro = oracle.fit(d, create_prior("none"), ctl)
+    bg, rg = E.fit_cyclops_model(d, create_prior("none"), ctl, precision=64, mode=mode)
+    assert abs(bo[-1]) > 20 and abs(bg[-1]) > 20   # the separable coefficient has run off
+    assert rg["return_flag"] == ro["return_flag"]
+    assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-6 * abs(ro["log_likelihood"])
+    if conv == "mittal":   # stops early enough that the steps still agree
+        assert rg["iterations"] == ro["iterations"] and _rel(bg, bo) <= 1e-6, (bg, bo)
