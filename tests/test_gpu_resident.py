@@ -242,3 +242,45 @@ def test_separable_logistic_follows_the_oracle(E, mode, conv):
     assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-6 * abs(ro["log_likelihood"])
     if conv == "mittal":   # stops early enough that the steps still agree
         assert rg["iterations"] == ro["iterations"] and _rel(bg, bo) <= 1e-6, (bg, bo)
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_degenerate_inputs_match_oracle(E, model):
+    """An empty column, a constant dense column and a tiny dataset (8 rows), through the
+    plain path, the resident kernel and a batch."""
+    from cyclops_amd import data as D
+    rng = np.random.default_rng(77)
+    for K in (8, 300):
+        X = (rng.random((K, 4)) < 0.4).astype(float)
+        X[:, 1] = 0.0                      # an empty indicator column
+        Xd = np.column_stack([X, np.full(K, 0.5)])
+        fm = ["indicator"] * 4 + ["dense"]
+        if model == "lr":
+            d = D.from_dense(Xd, (rng.random(K) < 0.4).astype(float), "lr", formats=fm, add_intercept=True)
+        elif model == "pr":
+            d = D.from_dense(Xd, rng.poisson(2.0, K).astype(float), "pr", formats=fm, add_intercept=True)
+        elif model == "clr":
+            pid = np.repeat(np.arange(K // 4), 4).astype(np.int32)
+            y = np.zeros(K)
+            y[::4] = 1.0
+            d = D.from_dense(Xd, y, "clr", formats=fm, pid=pid)
+        else:
+            t = np.sort(rng.integers(1, 6, K))[::-1].astype(float)
+            y = (rng.random(K) < 0.5).astype(float)
+            order = np.lexsort((np.arange(K), y, -t))
+            d = D.from_dense(Xd[order], y[order], "cox", formats=fm, time=t[order], pid=np.zeros(K, dtype=np.int32))
+        print(f"[case] degenerate {model} K={K}", flush=True)
+        prior = create_prior("normal", 1.0)
+        ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+        bo, ro = oracle.fit(d, prior, ctl)
+        for mode in ("plain", "resident"):
+            bg, rg = E.fit_cyclops_model(d, prior, ctl, precision=64, mode=mode)
+            assert rg["return_flag"] == ro["return_flag"] and rg["iterations"] == ro["iterations"], (mode, rg, ro)
+            assert _rel(bg, bo) <= 1e-6, (mode, bg, bo)
+        eng = E.Engine(d, 64)
+        try:
+            bb, rb = eng.fit_batch([1.0, 1.0, 1.0], "normal", ctl)
+        finally:
+            eng.close()
+        for i in range(3):
+            assert _rel(bb[i], bo) <= 1e-6 and rb[i]["iterations"] == ro["iterations"]
