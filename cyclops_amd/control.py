@@ -34,11 +34,14 @@ class Control:
     seed: int = 1
     selector: str = "auto"               # auto | byRow | byPid
     reset_coefficients: bool = False     # R's default: cross-validation fits warm-start in a chain
+    min_cv_data: int = 100               # fewer rows (byRow) or strata (byPid) refuse cross-validation
 
 
 def create_prior(prior_type="none", variance=1.0, exclude=(), use_cross_validation=False) -> Prior:
     if prior_type not in _abi.PRIOR_NAMES:
         raise ValueError(f"unknown prior type {prior_type!r}")
+    if prior_type == "none" and use_cross_validation:   # R/ModelFit.R:596-598
+        raise ValueError("Cannot perform cross validation with a flat prior")
     return Prior(prior_type, float(variance), tuple(exclude), use_cross_validation)
 
 

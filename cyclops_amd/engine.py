@@ -327,9 +327,27 @@ class Engine:
 
 def fit_cyclops_model(data, prior=None, control=None, weights=None, fixed=None, beta0=None,
                       precision=_abi.FP64, mode="plain", device=0):
-    """One fit on the device: fitCyclopsModel (R/ModelFit.R:65-293)."""
+    """One fit on the device: fitCyclopsModel (R/ModelFit.R:65-293). With
+    prior.use_cross_validation the prior variance is chosen by cross-validation first
+    (:262-274, .cyclopsRunCrossValidation) and the result carries it as "variance"."""
+    prior = prior or create_prior("none")
+    control = control or create_control()
     eng = Engine(data, precision, device)
     try:
+        if prior.use_cross_validation:
+            selector = control.selector
+            if selector == "auto":   # R/ModelFit.R:174-188
+                selector = "byPid" if data.model_type == _abi.CONDITIONAL_LOGISTIC else "byRow"
+            count = data.K if (selector == "byRow" or data.pid is None) else len(np.unique(data.pid))
+            if control.min_cv_data > count:
+                raise CyclopsError("Insufficient data count for cross validation")
+            if weights is not None or fixed is not None or beta0 is not None:
+                raise CyclopsError("cross-validation here takes no weights, fixed or starting coefficients")
+            sel = _abi.SELECT_BY_PID if selector == "byPid" else _abi.SELECT_BY_ROW
+            out = eng.cross_validate(prior, control, sel)
+            res = dict(out["result"])
+            res["variance"] = out["best_variance"]
+            return out["beta"], res
         return eng.fit(prior, control, weights=weights, fixed=fixed, beta0=beta0, mode=mode)
     finally:
         eng.close()

@@ -50,3 +50,10 @@ def test_create_without_device_fails_cleanly():
     d = _cases.logistic()
     with pytest.raises(engine.CyclopsError):
         engine.Engine(d)
+
+
+def test_flat_prior_refuses_cross_validation():
+    import pytest
+    from cyclops_amd.control import create_prior
+    with pytest.raises(ValueError, match="flat prior"):
+        create_prior("none", use_cross_validation=True)

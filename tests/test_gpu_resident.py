@@ -284,3 +284,17 @@ def test_degenerate_inputs_match_oracle(E, model):
             eng.close()
         for i in range(3):
             assert _rel(bb[i], bo) <= 1e-6 and rb[i]["iterations"] == ro["iterations"]
+
+
+def test_fit_cyclops_model_with_cross_validation(E):
+    """prior.use_cross_validation routes fitCyclopsModel through the CV driver."""
+    d = _cases.logistic(K=400, seed=31)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8, fold=4, grid_size=4, lower_limit=0.01,
+                         upper_limit=2.0, seed=7, reset_coefficients=True)
+    prior = create_prior("laplace", 1.0, use_cross_validation=True)
+    b, r = E.fit_cyclops_model(d, prior, ctl)
+    ro = oracle.cross_validate(d, prior, ctl, _abi.SELECT_BY_ROW)
+    assert r["variance"] == pytest.approx(ro["best_variance"], rel=1e-12)
+    assert _rel(b, ro["beta"]) <= 1e-6
+    with pytest.raises(E.CyclopsError, match="Insufficient"):
+        E.fit_cyclops_model(_cases.logistic(K=60, seed=3), prior, ctl)
