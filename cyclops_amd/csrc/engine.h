@@ -10,6 +10,7 @@
 #include "../../include/cyclops_amd.h"
 
 #include <memory>
+#include <mutex>
 #include <vector>
 
 namespace ccd {
@@ -40,6 +41,13 @@ struct DevData {
     int32_t* dStrat = nullptr;
     int32_t* dStratStart = nullptr;
     double bytes = 0.0;
+    // Row-major copy of the INDICATOR/SPARSE entries, built on first use: the entries
+    // of row k are [dRowPtr[k], dRowPtr[k+1]), ascending in column (warm-started xb).
+    std::mutex rowsMu;
+    int64_t* dRowPtr = nullptr;
+    int32_t* dRowCol = nullptr;
+    Real* dRowVal = nullptr;
+    void buildRowMajor(hipStream_t s);
 
     ~DevData();
 };

@@ -86,6 +86,51 @@ DevData<Real>::~DevData() {
     dfree(dDensePtr);
     dfree(dStrat);
     dfree(dStratStart);
+    dfree(dRowPtr);
+    dfree(dRowCol);
+    dfree(dRowVal);
+}
+
+// Counting sort of the column-major entries by row, columns visited in order, so
+// every row's entries stay ascending in column.
+template <typename Real>
+void DevData<Real>::buildRowMajor(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(rowsMu);
+    if (dRowPtr) return;
+    std::vector<int32_t> rows(nnz);
+    std::vector<Real> vals;
+    if (nnz) CCD_HIP(hipMemcpyAsync(rows.data(), dRows, nnz * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (dVals && nnz) {
+        vals.resize(nnz);
+        CCD_HIP(hipMemcpyAsync(vals.data(), dVals, nnz * sizeof(Real), hipMemcpyDeviceToHost, s));
+    }
+    CCD_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> ptr(K + 1, 0);
+    for (int j = 0; j < J; ++j)
+        if (fmt[j] == F_INDICATOR || fmt[j] == F_SPARSE)
+            for (int64_t e = colPtr[j]; e < colPtr[j + 1]; ++e) ++ptr[rows[e] + 1];
+    for (int64_t k = 0; k < K; ++k) ptr[k + 1] += ptr[k];
+    const int64_t n = ptr[K];
+    std::vector<int64_t> next(ptr.begin(), ptr.end() - 1);
+    std::vector<int32_t> col(n);
+    std::vector<Real> val(n);
+    for (int j = 0; j < J; ++j) {
+        if (!(fmt[j] == F_INDICATOR || fmt[j] == F_SPARSE)) continue;
+        for (int64_t e = colPtr[j]; e < colPtr[j + 1]; ++e) {
+            const int64_t q = next[rows[e]]++;
+            col[q] = j;
+            val[q] = fmt[j] == F_SPARSE ? vals[e] : Real(1);
+        }
+    }
+    double b = 0.0;
+    dRowPtr = dalloc<int64_t>(K + 1, b);
+    dRowCol = dalloc<int32_t>(n, b);
+    dRowVal = dalloc<Real>(n, b);
+    h2d(dRowPtr, ptr.data(), K + 1, s);
+    h2d(dRowCol, col.data(), n, s);
+    h2d(dRowVal, val.data(), n, s);
+    CCD_HIP(hipStreamSynchronize(s));
+    bytes += b;
 }
 
 // Validate the description and copy it to HBM.

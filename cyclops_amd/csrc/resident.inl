@@ -100,6 +100,27 @@ __global__ __launch_bounds__(BLOCK) void kXjYFold(int J, int64_t K, int F, const
     }
 }
 
+// xb of every fit from the row-major copy of the INDICATOR/SPARSE columns: thread
+// (k, f) sums beta_f[j] * x_kj over row k's entries in column order and stores it
+// (DENSE/INTERCEPT columns are added afterwards, column by column).
+template <typename Real>
+__global__ __launch_bounds__(BLOCK) void kXBetaRows(int64_t K, int B, int Bt, int BP, int J,
+                                                   const int64_t* __restrict__ rowPtr,
+                                                   const int32_t* __restrict__ rowCol,
+                                                   const Real* __restrict__ rowVal,
+                                                   const double* __restrict__ beta, Real* __restrict__ xb) {
+    const int64_t total = K * (int64_t)B;
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < total; t += stride) {
+        const int64_t k = t / B;
+        const int f = (int)(t % B);
+        const double* bf = beta + (size_t)f * J;
+        Real acc = Real(0);
+        for (int64_t e = rowPtr[k]; e < rowPtr[k + 1]; ++e) acc += (Real)bf[rowCol[e]] * rowVal[e];
+        xb[(size_t)(f / Bt) * K * BP + k * BP + f % Bt] = acc;
+    }
+}
+
 // xb[k*BP + f] += beta_f * x_kj for every fit f (one column).
 template <typename Real, int FMT>
 __global__ __launch_bounds__(BLOCK) void kAxpyBatch(Col<Real> c, const double* __restrict__ beta, int J, int j,
@@ -526,12 +547,28 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     // xb = X beta for every fit (column order, as CyclicCoordinateDescent::computeXBeta);
     // team t's block is xb[t*K*BP ...], fit-minor inside the team
     CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)nTeams * K * BP * sizeof(Real), s));
-    for (int j = 0; j < J; ++j) {
-        bool any = false;
-        for (int f = 0; f < B && !any; ++f) any = betas[(size_t)f * J + j] != 0.0;
-        const Col<Real> c = E.col(j);
-        if (!any || c.n == 0) continue;
-        CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, W.beta, J, j, B, L.Bt, BP, K, W.xb)));
+    {
+        // warm starts with many non-zero sparse columns form the sparse part of xb in
+        // one pass over rows; otherwise (and for DENSE/INTERCEPT columns) one axpy per column
+        std::vector<char> live(J, 0);
+        int liveSparse = 0;
+        for (int j = 0; j < J; ++j) {
+            for (int f = 0; f < B && !live[j]; ++f) live[j] = betas[(size_t)f * J + j] != 0.0;
+            if (live[j] && (D.fmt[j] == F_INDICATOR || D.fmt[j] == F_SPARSE)) ++liveSparse;
+        }
+        const bool byRows = liveSparse > 32;
+        if (byRows) {
+            D.buildRowMajor(s);
+            kXBetaRows<Real><<<gridFor(K * (int64_t)B), BLOCK, 0, s>>>(K, B, L.Bt, BP, J, D.dRowPtr, D.dRowCol,
+                                                                      D.dRowVal, W.beta, W.xb);
+            CCD_HIP(hipGetLastError());
+        }
+        for (int j = 0; j < J; ++j) {
+            const Col<Real> c = E.col(j);
+            if (!live[j] || c.n == 0) continue;
+            if (byRows && (D.fmt[j] == F_INDICATOR || D.fmt[j] == F_SPARSE)) continue;
+            CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, W.beta, J, j, B, L.Bt, BP, K, W.xb)));
+        }
     }
     CCD_HIP(hipGetLastError());
     CCD_HIP(hipMemsetAsync(W.steps, 0, sizeof(unsigned long long), s));

@@ -246,3 +246,26 @@ def test_variance_entry_points_refuse_bad_arguments(E):
         assert lo[0] < b[1] < up[0]
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_warm_started_batch_over_many_columns_matches_oracle(E, precision):
+    """Starting values on 100 sparse columns: xb of every fit is formed from the
+    row-major copy of the columns in one pass."""
+    d, _ = D.make_config(0, scale=0.1)
+    print(f"[case] warm batch K={d.K} J={d.J} fp{precision}", flush=True)
+    rng = np.random.default_rng(9)
+    b0 = rng.normal(0, 0.2, (3, d.J))
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    eng = E.Engine(d, precision)
+    try:
+        bg, rg = eng.fit_batch([0.5, 1.0, 2.0], "normal", ctl, betas0=b0)
+    finally:
+        eng.close()
+    for i, v in enumerate([0.5, 1.0, 2.0]):
+        bo, ro = oracle.fit(d, create_prior("normal", v), ctl, beta0=b0[i])
+        if precision == 64:
+            assert rg[i]["iterations"] == ro["iterations"]
+            assert _rel(bg[i], bo) <= 1e-6, i
+        else:
+            assert _rel(bg[i], bo) <= 1e-4, i
