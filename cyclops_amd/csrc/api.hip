@@ -326,7 +326,7 @@ static void crossValidate(EngineBase& eng, const ccd_fit_args& a, const ccd_cv_a
     if (eng.hasOffset())
         for (int i = 0; i < nFits; ++i) betas[(size_t)i * J] = 1.0;
     std::vector<ccd_fit_result> res(nFits);
-    ccd_batch_args b{nFits, var.data(), held.data(), rowFold.data()};
+    ccd_batch_args b{nFits, var.data(), held.data(), rowFold.data(), nullptr};
     ccd_fit_args common = a;
     common.weights = nullptr;
     eng.fitBatch(common, b, betas.data(), res.data());

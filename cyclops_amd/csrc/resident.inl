@@ -94,7 +94,7 @@ __global__ __launch_bounds__(BLOCK) void kXjYFold(int J, int64_t K, int F, const
             double s = 0.0;
             for (int wv = 0; wv < WAVES; ++wv) s += red[threadIdx.x][wv];
             if (threadIdx.x == 32) tot[j] = s;
-            else if (threadIdx.x < F && byFold) byFold[(size_t)j * F + threadIdx.x] = s;
+            else if ((int)threadIdx.x < F && byFold) byFold[(size_t)j * F + threadIdx.x] = s;
         }
         __syncthreads();
     }

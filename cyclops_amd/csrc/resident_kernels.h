@@ -1260,7 +1260,6 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     __shared__ Shared<BP> s;
     const int w = P.w;
-    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
     const unsigned long long deadline = P.deadlineTicks;   // per-wait limit
     unsigned seq = 0;
     const bool own = P.allRead != 0;   // control state in LDS (all-read) or global (owner mode)
