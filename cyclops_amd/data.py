@@ -175,6 +175,56 @@ def from_dense(X, y, model, formats=None, time=None, pid=None, add_intercept=Fal
                    has_offset=offset is not None, names=nm)
 
 
+def _csc_slots(m, K, with_values):
+    """(i, p[, x]) of a column-compressed matrix given as a scipy sparse matrix or as the
+    dgCMatrix slots themselves (0-based row ids i, column pointers p)."""
+    if hasattr(m, "tocsc"):
+        c = m.tocsc()
+        c.sort_indices()
+        if c.shape[0] != K:
+            raise ValueError("sparse matrix rows must equal len(y)")
+        return c.indices.astype(np.int32), c.indptr.astype(np.int64), (c.data.astype(np.float64) if with_values else None)
+    if with_values:
+        i, pp, x = m
+        return np.asarray(i, dtype=np.int32), np.asarray(pp, dtype=np.int64), np.asarray(x, dtype=np.float64)
+    i, pp = m[0], m[1]
+    return np.asarray(i, dtype=np.int32), np.asarray(pp, dtype=np.int64), None
+
+
+def from_matrices(y, model, dense=None, sparse=None, indicator=None, time=None, pid=None, offset=None,
+                  names=None) -> CyclopsData:
+    """cyclopsModelData's column assembly (RcppModelData.cpp:671-737, 751-900): the offset
+    column (when given, beta fixed at 1), then the dense matrix's columns (K x Pd, row
+    major here), then the SPARSE columns of a column-compressed matrix (dgCMatrix slots
+    i, p, x or a scipy sparse matrix), then the INDICATOR columns of another (slots i, p;
+    every stored entry is 1)."""
+    model = _MODEL_FROM_NAME.get(model, model)
+    y = np.asarray(y, dtype=np.float64)
+    K = len(y)
+    formats, rows_l, vals_l, dense_l = [], [], [], []
+    if offset is not None:
+        formats.append(_abi.DENSE); rows_l.append(None); vals_l.append(None)
+        dense_l.append(np.asarray(offset, dtype=np.float64))
+    if dense is not None:
+        Xd = np.asarray(dense, dtype=np.float64).reshape(K, -1)
+        for q in range(Xd.shape[1]):
+            formats.append(_abi.DENSE); rows_l.append(None); vals_l.append(None); dense_l.append(Xd[:, q])
+    for m, fmt in ((sparse, _abi.SPARSE), (indicator, _abi.INDICATOR)):
+        if m is None:
+            continue
+        i, pp, x = _csc_slots(m, K, fmt == _abi.SPARSE)
+        for q in range(len(pp) - 1):
+            a, b = int(pp[q]), int(pp[q + 1])
+            formats.append(fmt)
+            rows_l.append(i[a:b])
+            vals_l.append(x[a:b] if x is not None else None)
+            dense_l.append(None)
+    if not formats:
+        raise ValueError("no covariates")
+    return _finish(model, K, formats, rows_l, vals_l, dense_l, y, time=time, pid=pid,
+                   has_offset=offset is not None, names=names)
+
+
 def read_generic_text(path, model, add_intercept=None, first_field="row", sep=None,
                       dense=False) -> CyclopsData:
     """Reader for the reference's generic text fixtures (io/NewGenericInputReader.h:16-217):

@@ -57,3 +57,27 @@ def test_flat_prior_refuses_cross_validation():
     from cyclops_amd.control import create_prior
     with pytest.raises(ValueError, match="flat prior"):
         create_prior("none", use_cross_validation=True)
+
+
+def test_from_matrices_assembles_like_the_reference():
+    """Offset, dense, sparse (dgCMatrix slots), indicator: the columns of one dataset built
+    from matrices equal the same columns built from a dense array."""
+    import numpy as np
+    import scipy.sparse as sp
+    from cyclops_amd import data as D
+    rng = np.random.default_rng(3)
+    K = 50
+    Xd = rng.normal(size=(K, 2))
+    Xs = np.where(rng.random((K, 3)) < 0.3, rng.integers(1, 5, (K, 3)), 0).astype(float)
+    Xi = (rng.random((K, 2)) < 0.3).astype(float)
+    y = rng.poisson(1.0, K).astype(float)
+    off = np.log(rng.uniform(1, 5, K))
+    s = sp.csc_matrix(Xs)
+    a = D.from_matrices(y, "pr", dense=Xd, sparse=(s.indices, s.indptr, s.data), indicator=sp.csc_matrix(Xi),
+                        offset=off)
+    b = D.from_dense(np.column_stack([Xd, Xs, Xi]), y, "pr",
+                     formats=["dense"] * 2 + ["sparse"] * 3 + ["indicator"] * 2, offset=off)
+    assert np.array_equal(a.formats, b.formats) and np.array_equal(a.col_ptr, b.col_ptr)
+    assert np.array_equal(a.rows, b.rows) and np.allclose(a.values, b.values)
+    assert np.array_equal(a.dense_values, b.dense_values) and a.has_offset and b.has_offset
+    assert np.array_equal(a.to_dense(), b.to_dense())
