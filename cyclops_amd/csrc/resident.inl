@@ -745,15 +745,20 @@ static void gatherRun(ResRun<Real>& R, double* betas, int* flags, int* iters, do
                       double& steps) {
     auto& W = *R.W;
     const int B = R.B, J = R.J;
+    // copies on the engine's own stream: a synchronous hipMemcpy goes through the null
+    // stream, whose hardware queue other engines' persistent launches may share, and
+    // would wait for them
+    hipStream_t s = R.E->s;
     unsigned long long st = 0;
-    CCD_HIP(hipMemcpy(&st, W.steps, sizeof(st), hipMemcpyDeviceToHost));
-    steps = (double)st;
-    CCD_HIP(hipMemcpy(betas, W.beta, (size_t)B * J * sizeof(double), hipMemcpyDeviceToHost));
-    CCD_HIP(hipMemcpy(flags, W.flag, B * sizeof(int), hipMemcpyDeviceToHost));
-    CCD_HIP(hipMemcpy(iters, W.iters, B * sizeof(int), hipMemcpyDeviceToHost));
-    CCD_HIP(hipMemcpy(lls, W.ll, B * sizeof(double), hipMemcpyDeviceToHost));
     double st8[8];
-    CCD_HIP(hipMemcpy(st8, W.stats, sizeof(st8), hipMemcpyDeviceToHost));
+    CCD_HIP(hipMemcpyAsync(&st, W.steps, sizeof(st), hipMemcpyDeviceToHost, s));
+    CCD_HIP(hipMemcpyAsync(betas, W.beta, (size_t)B * J * sizeof(double), hipMemcpyDeviceToHost, s));
+    CCD_HIP(hipMemcpyAsync(flags, W.flag, B * sizeof(int), hipMemcpyDeviceToHost, s));
+    CCD_HIP(hipMemcpyAsync(iters, W.iters, B * sizeof(int), hipMemcpyDeviceToHost, s));
+    CCD_HIP(hipMemcpyAsync(lls, W.ll, B * sizeof(double), hipMemcpyDeviceToHost, s));
+    CCD_HIP(hipMemcpyAsync(st8, W.stats, sizeof(st8), hipMemcpyDeviceToHost, s));
+    CCD_HIP(hipStreamSynchronize(s));
+    steps = (double)st;
     algoBytes = st8[0];
     if (std::getenv("CCD_PHASES"))
         std::fprintf(stderr, "[ccd] teams=%d T=%d B=%d BP=%d steps=%llu  ms (summed over teams): column pass %.1f  exchange %.1f  update %.1f  objective %.1f\n",
