@@ -170,6 +170,28 @@ def test_co_resident_teams_are_exact_at_scale(E, monkeypatch):
         assert np.max(np.abs(b - out["1,1"])) <= 1e-12, layout
 
 
+@pytest.mark.parametrize("workgroups", [1, 3, 8])
+@pytest.mark.parametrize("model", ["lr", "pr"])
+def test_single_fit_workgroup_count_matches_oracle(E, model, workgroups, monkeypatch):
+    """A single fit of independent rows on one workgroup (no exchange, the default up to
+    100k rows) and forced onto several (a grid exchange per coordinate): FP64
+    coefficients within 1e-6 of the oracle and the same cycles either way."""
+    d = _cases.ALL[model](K=3000, seed=31)
+    print(f"[case] single fit {model} on {workgroups} workgroup(s)", flush=True)
+    prior = _prior(model)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    monkeypatch.setenv("CCD_RESIDENT_WG", str(workgroups))
+    eng = E.Engine(d, 64)
+    try:
+        b, r = eng.fit(prior, ctl, mode="resident")
+    finally:
+        eng.close()
+    bo, ro = oracle.fit(d, prior, ctl)
+    assert r["iterations"] == ro["iterations"], (r, ro)
+    assert _rel(b, bo) <= 1e-6
+    assert abs(r["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
+
+
 @pytest.mark.parametrize("model", [2, 3])
 def test_grouped_multi_workgroup_matches_oracle(E, model):
     """Cox (config 2 generator) and CLR (config 3 generator) at sizes where the one-fit

@@ -21,7 +21,7 @@ def E():
 
 def _mix():
     ctl = create_control(convergence_type="mittal", tolerance=1e-8)
-    d0, _ = D.make_config(0, scale=0.5)   # 50,000 rows: a multi-workgroup persistent launch
+    d0, _ = D.make_config(0, scale=1.2)   # 120,000 rows: a multi-workgroup persistent launch
     return [
         (d0, create_prior("laplace", 0.1), ctl),
         (_cases.logistic(K=2000, seed=11), create_prior("laplace", 0.5), ctl),
