@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# Serving runs one stream per slot and needs a hardware queue for each (HIP reads
+# this once, at its first call in the process; the default is 4).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built library")

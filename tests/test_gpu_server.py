@@ -64,6 +64,35 @@ def test_results_same_alone_and_in_mix(E):
         assert np.max(np.abs(b - bo)) <= 1e-6 * max(np.max(np.abs(bo)), 1e-12)
 
 
+def test_short_requests_do_not_wait_for_a_long_launch(E, monkeypatch):
+    """A long fit held in one persistent launch (300 cycles, no convergence test) and
+    short requests arriving while it runs: the short ones return long before it ends.
+    Nothing in a request's path may wait on the device as a whole or on the null
+    stream, which shares a hardware queue with some slot."""
+    from cyclops_amd import _abi
+    import time
+    monkeypatch.setenv("CCD_CYCLES_PER_LAUNCH", "100000")
+    big = D.make_request(2, max_rows=100_000, max_cols=10_000)["data"]   # LR, 73,412 x 1,414
+    long_ctl = create_control(convergence_type="mittal", tolerance=0.0, max_iterations=300)
+    small = [_cases.logistic(K=400, seed=40 + i) for i in range(3)]
+    print(f"[case] server long launch K={big.K} J={big.J} + {len(small)} short requests", flush=True)
+    srv = E.Server(slots=2, hbm_budget_bytes=8e9)
+    try:
+        lid = srv.submit(big, create_prior("laplace", 0.1), long_ctl, precision=_abi.FP32)
+        time.sleep(0.5)
+        sids = [srv.submit(d, create_prior("laplace", 1.0), create_control()) for d in small]
+        short = [srv.poll(i, wait=True) for i in sids]
+        _, lr_, ls = srv.poll(lid, wait=True)
+    finally:
+        srv.stop()
+    assert lr_["iterations"] == 300
+    assert ls["end"] - ls["start"] >= 1.0, ls   # the launch is long enough to tell
+    for _, r, st in short:
+        assert r["return_flag"] == 0
+        assert st["start"] >= ls["start"], (st, ls)
+        assert st["end"] < ls["end"] - 0.5, (st, ls)
+
+
 def _block(footprint):   # device block the server reserves for a request
     return ((int(footprint * 1.25) + (1 << 20)) + 255) & ~255
 
