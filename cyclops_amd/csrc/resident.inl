@@ -176,6 +176,10 @@ struct ResWork {
     double* stats = nullptr;
     Real* kw = nullptr;
     res::Params<Real>* teams = nullptr;
+    // parameter blocks of fused launches led by this engine (grow-only, so repeated
+    // fused launches call no hipMalloc / hipFree, which wait for the whole device)
+    res::Params<Real>* fusedTeams = nullptr;
+    size_t fusedCap = 0;
     std::map<int, std::array<int, 3>> tuned;   // batch size -> {Bt, T, nTeams} picked by timing
 
     void freeBatch() {
@@ -184,6 +188,8 @@ struct ResWork {
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
         dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw); dfree(teams); dfree(info);
+        dfree(fusedTeams);
+        fusedCap = 0;
         B = BP = nTeams = 0;
     }
     void freeAll() {
@@ -660,12 +666,17 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
         exchanges = exchanges || R->G > 1;
     }
     res::Params<Real>* dTeams = W0.teams;
-    double fb = 0.0;
     if (fused) {
         for (auto* R : runs) CCD_HIP(hipStreamSynchronize(R->E->s));   // every run's set-up is on the device
         for (auto* R : runs)
             for (auto& Q : R->teamP) { Q.err = W0.err; Q.allDoneOut = W0.allDone; }
-        dTeams = dalloc<res::Params<Real>>(total, fb);
+        if (W0.fusedCap < (size_t)total) {
+            dfree(W0.fusedTeams);
+            W0.fusedCap = 0;
+            W0.fusedTeams = dalloc<res::Params<Real>>(total, W0.bytes);
+            W0.fusedCap = (size_t)total;
+        }
+        dTeams = W0.fusedTeams;
     }
     std::vector<res::Params<Real>> all;
     auto upload = [&]() {
@@ -719,7 +730,6 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
         CCD_HIP(hipMemcpyAsync(&hs[1], W0.allDone, sizeof(int), hipMemcpyDeviceToHost, s));
         CCD_HIP(hipStreamSynchronize(s));
         if (hs[0]) {
-            if (fused) dfree(dTeams);
             throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
         }
         done = hs[1];
@@ -741,7 +751,6 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
     CCD_HIP(hipEventElapsedTime(&ms, E0.ev0, E0.ev1));
     deviceMs = ms;
     nLaunches = launches;
-    if (fused) dfree(dTeams);
 }
 
 // Results of a finished run.
