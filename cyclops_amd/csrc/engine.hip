@@ -757,8 +757,7 @@ namespace ccd {
 
 ResidentNeeds residentNeeds(const ccd_data_desc& d, int device) {
     ResidentNeeds n{};
-    const bool grp = d.model_type == CCD_MODEL_CONDITIONAL_LOGISTIC || d.model_type == CCD_MODEL_COX;
-    const int G = chooseWorkgroups(d.K, 1, device, grp);
+    const int G = chooseWorkgroups(d.K, 1, device, d.model_type);
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const int occ = d.precision == CCD_FP32 ? occupancyOf<float>(d.model_type, 1) : occupancyOf<double>(d.model_type, 1);

@@ -214,8 +214,9 @@ void EngineT<Real>::releaseResident() {
 // One or two fits are bound by the latency of the per-coordinate exchange, which
 // grows with the number of workgroups; batches are bound by bandwidth and use the
 // whole chip. Grouped models (CLR, Cox) also walk their per-pid state after every
-// update, so they take 4x more workgroups.
-static int chooseWorkgroups(int64_t K, int B, int device, bool grouped = false) {
+// update, so they take 4x more workgroups (a single Cox fit 16x).
+static int chooseWorkgroups(int64_t K, int B, int device, int model) {
+    const bool grouped = model == M_CLR || model == M_COX;
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const int cap = std::min(res::MAXG, cus);
@@ -228,7 +229,10 @@ static int chooseWorkgroups(int64_t K, int B, int device, bool grouped = false) 
     // workgroups (LR/PR, 16k-87k rows, 1.4k-8.8k columns). Grouped models keep gaining
     // from more workgroups at these sizes (their risk-set rescan is split over them).
     if (B == 1 && !grouped && K <= 100000) return 1;
-    const int64_t rowsPerWG = (B <= 2 && !grouped) ? 16384 : 4096;
+    // One Cox fit: every coordinate rescans all risk-set positions, split over the
+    // workgroups; about 1k rows per workgroup was fastest (11k-43k rows: 1.4-2.5x
+    // less per cycle than 4k rows per workgroup). CLR was flat from 6 to 48 workgroups.
+    const int64_t rowsPerWG = (B == 1 && model == M_COX) ? 1024 : (B <= 2 && !grouped) ? 16384 : 4096;
     int64_t g = (K + rowsPerWG - 1) / rowsPerWG;
     if (g < 1) g = 1;
     if (g > cap) g = cap;
@@ -244,13 +248,13 @@ struct Layout {
 };
 
 // CCD_BATCH_LAYOUT="Bt,T" overrides the team shape of batches (B > 2).
-static Layout chooseLayout(int64_t K, int B, int device, bool grouped = false) {
+static Layout chooseLayout(int64_t K, int B, int device, int model) {
     Layout L;
     int bt = 0, t = 0;
     if (const char* e = std::getenv("CCD_BATCH_LAYOUT")) std::sscanf(e, "%d,%d", &bt, &t);
     if (B <= 2) {   // one or two fits: the whole chip in lockstep
         L.Bt = B;
-        L.T = chooseWorkgroups(K, B, device, grouped);
+        L.T = chooseWorkgroups(K, B, device, model);
     } else if (bt > 0 && t > 0) {
         L.Bt = std::min(bt, 256);
         L.T = std::min(t, res::MAXG);
@@ -321,7 +325,7 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
     if (grouped && B <= 16)
         for (int t : {64, 128})
             if (K / t >= 2048) add(1, t);
-    if (B <= 256) add(B, chooseWorkgroups(K, B, device, grouped));
+    if (B <= 256) add(B, chooseWorkgroups(K, B, device, model));
     for (int bt : {4, 16, 64}) {
         if (bt >= B) continue;
         for (int t : {8, 16, 24, 32, 48, 64, 96, 128})
@@ -809,7 +813,7 @@ void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_res
         int nl = 0;
         runResident<Real>(*this, a, 1, &var, nullptr, nullptr, 0, a.weights, &initBound, b.data(), &flag, &it, &ll,
                           ms, bytes, steps, nl,
-                          chooseLayout(D->K, 1, D->device, D->model == M_CLR || D->model == M_COX));
+                          chooseLayout(D->K, 1, D->device, D->model));
         ++count;
         if (timing) {
             counters.deviceMs += ms;
@@ -917,11 +921,11 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     while (!pending.empty()) {
         std::vector<int> retry;
         const bool whole = tunedOk && (int)pending.size() == n;
-        const Layout L0 = whole ? LT : chooseLayout(D->K, (int)pending.size(), D->device, grouped);
+        const Layout L0 = whole ? LT : chooseLayout(D->K, (int)pending.size(), D->device, D->model);
         const size_t chunk = whole ? (size_t)n : (size_t)chunkFits(L0, D->K, nGroups, grouped, (int)sizeof(Real));
         for (size_t c0 = 0; c0 < pending.size(); c0 += chunk) {
             const int B = (int)std::min(chunk, pending.size() - c0);
-            const Layout L = whole ? LT : chooseLayout(D->K, B, D->device, grouped);
+            const Layout L = whole ? LT : chooseLayout(D->K, B, D->device, D->model);
             std::vector<double> v(B), bd(B), bb((size_t)B * J), l(B);
             std::vector<int32_t> h(B), fj(B);
             std::vector<int> fl(B), it(B);
@@ -1007,7 +1011,7 @@ static void fitMultiT(const std::vector<EngineT<Real>*>& engs, const ccd_fit_arg
         if (engs[i]->D->hasOffset) b[i][0] = 1.0;
         bound[i] = args[i].initial_bound;
         var[i] = args[i].variance;
-        lay[i] = chooseLayout(engs[i]->D->K, 1, device, grouped);   // the shape of its own resident fit
+        lay[i] = chooseLayout(engs[i]->D->K, 1, device, model);   // the shape of its own resident fit
         if (lay[i].T > capacity) throw std::invalid_argument("a fit needs more workgroups than the device holds");
     }
     std::vector<int> pending(n);
