@@ -325,6 +325,18 @@ int ccd_server_submit(ccd_server_handle s, const ccd_data_desc* d, const ccd_fit
             const ccd::ResidentNeeds n = ccd::residentNeeds(r->desc, s->cfg.device);
             bytes += n.workspaceBytes;
             r->capFraction = n.capacityFraction;
+            // starting values on more than 32 SPARSE/INDICATOR columns: the fit forms xb
+            // from a row-major copy of those columns (row pointers, column ids, values)
+            int live = 0;
+            int64_t idxNnz = 0;
+            for (int j = 0; j < J; ++j) {
+                const int f = r->format[j];
+                if (f != CCD_FORMAT_SPARSE && f != CCD_FORMAT_INDICATOR) continue;
+                idxNnz += r->colPtr[j + 1] - r->colPtr[j];
+                if ((j == 0 && d->has_offset) || r->beta[j] != 0.0) ++live;
+            }
+            if (live > 32)
+                bytes += (double)(K + 1) * 8.0 + (double)idxNnz * (4.0 + (d->precision == CCD_FP32 ? 4.0 : 8.0));
         }
         r->footprint = bytes;
         r->blockSize = blockBytes(bytes);

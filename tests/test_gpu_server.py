@@ -93,6 +93,34 @@ def test_short_requests_do_not_wait_for_a_long_launch(E, monkeypatch):
         assert st["end"] < ls["end"] - 0.5, (st, ls)
 
 
+def test_warm_start_on_many_sparse_columns_is_charged_and_exact(E):
+    """Starting values on more than 32 sparse columns: the admission footprint adds the
+    row-major copy the fit builds for xb, and the served result equals the same fit
+    run alone."""
+    d, _ = D.make_config(0, scale=0.1)   # 10,000 rows, 100 columns
+    idx = [j for j in range(d.J) if int(d.formats[j]) in (1, 2)]   # SPARSE / INDICATOR
+    assert len(idx) > 32
+    b0 = np.zeros(d.J)
+    b0[idx] = np.random.default_rng(7).normal(0, 0.2, len(idx))
+    prior, ctl = create_prior("normal", 1.0), create_control(convergence_type="mittal", tolerance=1e-8)
+    print(f"[case] server warm start on {len(idx)} sparse columns, K={d.K}", flush=True)
+    eng = E.Engine(d, 64)
+    try:
+        b_alone, r_alone = eng.fit(prior, ctl, beta0=b0, mode="resident")
+    finally:
+        eng.close()
+    srv = E.Server(slots=2, hbm_budget_bytes=8e9)
+    try:
+        cold = srv.poll(srv.submit(d, prior, ctl), wait=True)
+        b, r, st = srv.poll(srv.submit(d, prior, ctl, beta0=b0), wait=True)
+    finally:
+        srv.stop()
+    nnz = sum(int(d.col_ptr[j + 1] - d.col_ptr[j]) for j in idx)
+    assert st["footprint_bytes"] - cold[2]["footprint_bytes"] == (d.K + 1) * 8 + nnz * (4 + 8)
+    assert np.array_equal(b, b_alone)
+    assert r["iterations"] == r_alone["iterations"] and r["log_likelihood"] == r_alone["log_likelihood"]
+
+
 def _block(footprint):   # device block the server reserves for a request
     return ((int(footprint * 1.25) + (1 << 20)) + 255) & ~255
 
