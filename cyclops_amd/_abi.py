@@ -29,7 +29,22 @@ CONV_NAMES = {"gradient": CONV_GRADIENT, "lange": CONV_LANGE, "mittal": CONV_MIT
 # UpdateReturnFlags
 SUCCESS, FAIL, MAX_ITERATIONS, ILLCONDITIONED = 0, 1, 2, 3
 
-FP64, FP32 = 64, 32
+# PrecisionType (src/cyclops/Types.h:155-160): FP64 = 0, FP32 = 1. The library also
+# accepts the bit counts 64 and 32.
+FP64, FP32 = 0, 1
+FP64_BITS, FP32_BITS = 64, 32
+
+
+
+def precision_of(p):
+    """FP64 / FP32 for a PrecisionType value or a bit count (64 / 32)."""
+    if p in (FP64, FP64_BITS):
+        return FP64
+    if p in (FP32, FP32_BITS):
+        return FP32
+    raise ValueError(f"precision must be FP64 ({FP64}) or FP32 ({FP32}), or 64 / 32; got {p!r}")
+
+
 MODE_PLAIN, MODE_RESIDENT = 0, 1
 SELECT_BY_ROW, SELECT_BY_PID = 0, 1
 

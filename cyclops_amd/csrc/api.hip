@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <random>
 #include <string>
 #include <vector>
@@ -14,6 +15,11 @@
 struct ccd_engine {
     ccd::EngineBase* e = nullptr;
     ~ccd_engine() { delete e; }
+};
+
+struct ccd_batch_run {
+    ccd_engine* h = nullptr;
+    std::unique_ptr<ccd::SteppedBatch> b;
 };
 
 namespace {
@@ -372,6 +378,12 @@ static void crossValidate(EngineBase& eng, const ccd_fit_args& a, const ccd_cv_a
 
 }  // namespace ccd
 
+// Plain copy: 16-byte loads and stores, grid-stride (bandwidth measurement).
+__global__ __launch_bounds__(256) void kCopy16(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -410,7 +422,11 @@ int ccd_clone(ccd_handle h, ccd_handle* out) {
 }
 
 int ccd_destroy(ccd_handle h) {
-    return guard([&] { delete h; });
+    return guard([&] {
+        if (h && h->e && h->e->steppedOpen())
+            throw std::runtime_error("a stepped batch is open on this engine: end it first");
+        delete h;
+    });
 }
 
 int ccd_get_dims(ccd_handle h, int64_t* K, int32_t* J, int64_t* N) {
@@ -510,6 +526,46 @@ int ccd_fit_batch(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args
     });
 }
 
+int ccd_batch_begin(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args* batch, const double* betas0,
+                    ccd_batch_handle* out) {
+    return guard([&] {
+        if (!common || !batch || !betas0 || !out) throw std::invalid_argument("null argument");
+        if (common->weights) throw std::invalid_argument("batched fits take weights from row_fold");
+        if (batch->n_fits < 1) throw std::invalid_argument("n_fits must be >= 1");
+        ccd::checkArgs(E(h), *common);
+        std::unique_ptr<ccd_batch_run> r(new ccd_batch_run());
+        r->h = h;
+        r->b.reset(E(h).beginStepped(*common, *batch, betas0));
+        *out = r.release();
+    });
+}
+
+int ccd_batch_step(ccd_batch_handle b, int32_t cycles, int32_t* done, int32_t* running) {
+    return guard([&] {
+        if (!b || !b->b) throw std::invalid_argument("null batch handle");
+        const int n = b->b->step(cycles, done);
+        if (running) *running = n;
+    });
+}
+
+int ccd_batch_end(ccd_batch_handle b, double* betas, ccd_fit_result* results) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("null batch handle");
+        std::unique_ptr<ccd_batch_run> own(b);
+        if (!betas && !results) return;   // abandoned: the destructor releases the engine
+        if (!betas || !results) throw std::invalid_argument("betas and results go together");
+        std::memset(results, 0, sizeof(ccd_fit_result) * (size_t)own->b->fits());
+        own->b->finish(betas, results);
+    });
+}
+
+int ccd_batch_layout(ccd_batch_handle b, int32_t* fits_per_team, int32_t* workgroups_per_team, int32_t* teams) {
+    return guard([&] {
+        if (!b || !b->b || !fits_per_team || !workgroups_per_team || !teams) throw std::invalid_argument("null argument");
+        b->b->layout(fits_per_team, workgroups_per_team, teams);
+    });
+}
+
 int ccd_fit_multi(const ccd_handle* handles, int32_t n, const ccd_fit_args* args, double* const* betas,
                   ccd_fit_result* results) {
     return guard([&] {
@@ -593,10 +649,51 @@ int ccd_reset_kernel_stats(ccd_handle h) {
     });
 }
 
+int ccd_copy_bandwidth(int device, double bytes, int32_t reps, double* gbs) {
+    return guard([&] {
+        if (!gbs || !(bytes >= 16.0) || reps < 1) throw std::invalid_argument("bad arguments");
+        CCD_HIP(hipSetDevice(device));
+        const int64_t n = (int64_t)(bytes / 16.0);
+        float4 *a = nullptr, *b = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&]() {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+        };
+        try {
+            CCD_HIP(hipMalloc(&a, (size_t)n * 16));
+            CCD_HIP(hipMalloc(&b, (size_t)n * 16));
+            CCD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            CCD_HIP(hipEventCreate(&e0));
+            CCD_HIP(hipEventCreate(&e1));
+            CCD_HIP(hipMemsetAsync(a, 0, (size_t)n * 16, s));
+            const int blocks = (int)std::min<int64_t>(8192, (n + 255) / 256);
+            kCopy16<<<blocks, 256, 0, s>>>(a, b, n);
+            CCD_HIP(hipGetLastError());
+            CCD_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) kCopy16<<<blocks, 256, 0, s>>>(a, b, n);
+            CCD_HIP(hipGetLastError());
+            CCD_HIP(hipEventRecord(e1, s));
+            CCD_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CCD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            *gbs = 2.0 * (double)n * 16.0 * reps / ((double)ms * 1e-3) / 1e9;
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+    });
+}
+
 int ccd_estimate_footprint(const ccd_data_desc* d, double* bytes) {
     return guard([&] {
         if (!d || !bytes) throw std::invalid_argument("null argument");
-        const double s = d->precision == CCD_FP32 ? 4.0 : 8.0;
+        const double s = ccd::precisionOf(d->precision) == CCD_FP32 ? 4.0 : 8.0;
         const double nnz = (double)d->col_ptr[d->J];
         double nDense = 0;
         bool sparse = false;

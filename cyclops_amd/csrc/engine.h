@@ -56,6 +56,22 @@ struct KernelCounters {
     double launches = 0, deviceMs = 0, bytes = 0, coordinates = 0;
 };
 
+// A batch of fits advanced by the caller a given number of CCD cycles at a time
+// (ccd_batch_begin / ccd_batch_step / ccd_batch_end; resident.inl).
+class SteppedBatch {
+public:
+    virtual ~SteppedBatch() = default;
+    // Runs up to `cycles` more cycles of every unfinished fit; doneOut[f] (optional)
+    // is 1 once fit f has converged or stopped. Returns the fits still running.
+    virtual int step(int cycles, int32_t* doneOut) = 0;
+    // Runs the remaining cycles, applies update()'s bound retries and returns the
+    // coefficients and results (as ccd_fit_batch).
+    virtual void finish(double* betas, ccd_fit_result* r) = 0;
+    virtual int fits() const = 0;
+    virtual int cycles() const = 0;
+    virtual void layout(int32_t* fitsPerTeam, int32_t* workgroupsPerTeam, int32_t* teams) const = 0;
+};
+
 class EngineBase {
 public:
     virtual ~EngineBase() = default;
@@ -96,6 +112,10 @@ public:
     // issuing many small batches of varying size, such as profiling, pass false)
     virtual void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas,
                           ccd_fit_result* r, bool autotune = true) = 0;
+    // The same fits as fitBatch, stepped by the caller; the engine runs no other
+    // resident fit until the returned batch has finished or is deleted.
+    virtual SteppedBatch* beginStepped(const ccd_fit_args& a, const ccd_batch_args& b, const double* betas0) = 0;
+    virtual bool steppedOpen() const = 0;
 
     // Fisher information of columns idx[0..m) at beta, unit row weights (the engine's
     // weights are reset): out[m][m]. fisher.inl.
@@ -106,6 +126,14 @@ public:
 };
 
 EngineBase* createEngine(const ccd_data_desc& d, int device);
+
+// PrecisionType of a data description (FP64 = 0, FP32 = 1, or the bit counts 64 / 32)
+// as CCD_FP64 / CCD_FP32; anything else is refused.
+inline int precisionOf(int p) {
+    if (p == CCD_FP64 || p == CCD_FP64_BITS) return CCD_FP64;
+    if (p == CCD_FP32 || p == CCD_FP32_BITS) return CCD_FP32;
+    throw std::invalid_argument("precision must be CCD_FP64 (0) or CCD_FP32 (1); 64 and 32 are accepted too");
+}
 
 // What a one-fit persistent launch over `d` needs on `device`: its workgroups, the
 // fraction of the device's co-resident capacity they take, and the HBM of its

@@ -720,6 +720,8 @@ public:
     void fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) override;
     void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas, ccd_fit_result* r,
                   bool autotune) override;
+    SteppedBatch* beginStepped(const ccd_fit_args& a, const ccd_batch_args& b, const double* betas0) override;
+    bool steppedOpen() const override { return stepped != nullptr; }
     // fisher.inl
     void fisherInformation(const double* beta, const int32_t* idx, int m, double* out) override;
     void releaseResident();
@@ -745,6 +747,7 @@ public:
     double fixedTerm = 0.0;
     // resident-fit workspace (resident.hip)
     void* res = nullptr;
+    SteppedBatch* stepped = nullptr;   // open stepped batch using `res`, if any
     Arena* arena = nullptr;   // the creating thread's arena, if any (its stream and host buffer)
 };
 
@@ -760,10 +763,10 @@ ResidentNeeds residentNeeds(const ccd_data_desc& d, int device) {
     const int G = chooseWorkgroups(d.K, 1, device, d.model_type);
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    const int occ = d.precision == CCD_FP32 ? occupancyOf<float>(d.model_type, 1) : occupancyOf<double>(d.model_type, 1);
+    const int occ = precisionOf(d.precision) == CCD_FP32 ? occupancyOf<float>(d.model_type, 1) : occupancyOf<double>(d.model_type, 1);
     n.workgroups = G;
     n.capacityFraction = (double)G / std::max(1, occ * cus);
-    const double s = d.precision == CCD_FP32 ? 4.0 : 8.0;
+    const double s = precisionOf(d.precision) == CCD_FP32 ? 4.0 : 8.0;
     const bool grouped = d.model_type == CCD_MODEL_CONDITIONAL_LOGISTIC || d.model_type == CCD_MODEL_COX;
     n.workspaceBytes = d.K * (s + 12.0) + (grouped ? (d.K + 1) * 5.0 * 8.0 : 0.0) +
                        (double)d.J * ((G + 1) * 4.0 + G * 16.0 + 32.0 * 8.0 + 64.0) + (1 << 20);
@@ -771,9 +774,8 @@ ResidentNeeds residentNeeds(const ccd_data_desc& d, int device) {
 }
 
 EngineBase* createEngine(const ccd_data_desc& d, int device) {
-    if (d.precision == CCD_FP64) return new EngineT<double>(makeDevData<double>(d, device));
-    if (d.precision == CCD_FP32) return new EngineT<float>(makeDevData<float>(d, device));
-    throw std::invalid_argument("precision must be 64 or 32");
+    if (precisionOf(d.precision) == CCD_FP64) return new EngineT<double>(makeDevData<double>(d, device));
+    return new EngineT<float>(makeDevData<float>(d, device));
 }
 
 }  // namespace ccd

@@ -367,6 +367,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     const int J = D.J;
     const int m = D.model;
     const bool grouped = (m == M_CLR || m == M_COX);
+    if (E.stepped) throw std::runtime_error("a stepped batch is open on this engine: finish it first");
     if (!E.res) E.res = new ResWork<Real>();
     auto& W = *static_cast<ResWork<Real>*>(E.res);
     const int G = L.T;
@@ -616,6 +617,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     // when the partials to read per coordinate are few (G x fits), else the owner of
     // each fit steps it and broadcasts the delta (a second exchange)
     P.allRead = (L.Bt <= 2 || G == 1 || (!grouped && (int64_t)G * BP <= 1024)) ? 1 : 0;
+
     std::vector<res::Params<Real>> teamP(nTeams, P);
     for (int t = 0; t < nTeams; ++t) {
         auto& Q = teamP[t];
@@ -650,43 +652,49 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     return R;
 }
 
-// Launch prepared runs until all their fits have finished. One run launches its
-// own grid; several runs (same model, precision and fit padding) share ONE grid,
-// each run's teams keeping their own sizes, exchanges and state, and one error and
-// completion flag. Each launch runs up to `budget` cycles; state stays in HBM.
+// Launches of prepared runs. One run launches its own grid; several runs (same
+// model, precision and fit padding) share ONE grid, each run's teams keeping their
+// own sizes, exchanges and state, and one error and completion flag. Each launch
+// runs up to `budget` cycles of every unfinished fit and leaves the state in HBM,
+// so where the launches break changes no result.
 template <typename Real>
-static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs, int& nLaunches) {
-    ResRun<Real>& R0 = *runs[0];
-    auto& W0 = *R0.W;
-    EngineT<Real>& E0 = *R0.E;
-    hipStream_t s = E0.s;
-    const bool fused = runs.size() > 1;
-    int total = 0, fitsAll = 0;
-    bool exchanges = false;
-    for (auto* R : runs) {
-        if (R->m != R0.m || R->BP != R0.BP) throw std::logic_error("fused runs need one kernel instance");
-        total += R->nTeams * R->G;
-        fitsAll += R->B;
-        exchanges = exchanges || R->G > 1;
-    }
-    res::Params<Real>* dTeams = W0.teams;
-    if (fused) {
-        for (auto* R : runs) CCD_HIP(hipStreamSynchronize(R->E->s));   // every run's set-up is on the device
-        for (auto* R : runs)
-            for (auto& Q : R->teamP) { Q.err = W0.err; Q.allDoneOut = W0.allDone; }
-        if (W0.fusedCap < (size_t)total) {
-            dfree(W0.fusedTeams);
-            W0.fusedCap = 0;
-            W0.fusedTeams = dalloc<res::Params<Real>>(total, W0.bytes);
-            W0.fusedCap = (size_t)total;
-        }
-        dTeams = W0.fusedTeams;
-    }
+struct Launcher {
+    std::vector<ResRun<Real>*> runs;
+    res::Params<Real>* dTeams = nullptr;
+    bool fused = false, exchanges = false;
+    int total = 0, fitsAll = 0, launches = 0, lastBudget = -1;
     std::vector<res::Params<Real>> all;
-    auto upload = [&]() {
+
+    explicit Launcher(const std::vector<ResRun<Real>*>& rs) : runs(rs) {
+        ResRun<Real>& R0 = *runs[0];
+        auto& W0 = *R0.W;
+        fused = runs.size() > 1;
+        for (auto* R : runs) {
+            if (R->m != R0.m || R->BP != R0.BP) throw std::logic_error("fused runs need one kernel instance");
+            total += R->nTeams * R->G;
+            fitsAll += R->B;
+            exchanges = exchanges || R->G > 1;
+        }
+        dTeams = W0.teams;
+        if (fused) {
+            for (auto* R : runs) CCD_HIP(hipStreamSynchronize(R->E->s));   // every run's set-up is on the device
+            for (auto* R : runs)
+                for (auto& Q : R->teamP) { Q.err = W0.err; Q.allDoneOut = W0.allDone; }
+            if (W0.fusedCap < (size_t)total) {
+                dfree(W0.fusedTeams);
+                W0.fusedCap = 0;
+                W0.fusedTeams = dalloc<res::Params<Real>>(total, W0.bytes);
+                W0.fusedCap = (size_t)total;
+            }
+            dTeams = W0.fusedTeams;
+        }
+    }
+
+    void upload(hipStream_t s) {
+        ResRun<Real>& R0 = *runs[0];
         if (!fused) {
             R0.fill();
-            h2d(W0.teams, R0.blockP.data(), R0.blockP.size(), s);
+            h2d(R0.W->teams, R0.blockP.data(), R0.blockP.size(), s);
             return;
         }
         all.clear();
@@ -695,19 +703,15 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
             all.insert(all.end(), R->blockP.begin(), R->blockP.end());
         }
         h2d(dTeams, all.data(), all.size(), s);
-    };
-    // Cycles per launch: CCD_CYCLES_PER_LAUNCH, else adapted so a launch lasts about
-    // 4 s (the host sees progress and errors at launch boundaries): 2 cycles first,
-    // then as many as the measured cycle time allows, at most 32.
-    int budget = 2;
-    bool fixedBudget = false;
-    if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) { budget = std::max(1, std::atoi(e)); fixedBudget = true; }
-    const bool progress = std::getenv("CCD_PROGRESS") != nullptr;
-    const auto tWall = std::chrono::steady_clock::now();
-    CCD_HIP(hipEventRecord(E0.ev0, s));
-    int done = 0, launches = 0, cyclesRun = 0, lastBudget = -1;
-    const int one = 1;
-    while (!done) {
+    }
+
+    // One launch of up to `budget` cycles; true once every fit has finished. With
+    // `evA`/`evB` the kernel alone is bracketed by those events on the stream.
+    bool launch(int budget, hipEvent_t evA = nullptr, hipEvent_t evB = nullptr) {
+        ResRun<Real>& R0 = *runs[0];
+        auto& W0 = *R0.W;
+        hipStream_t s = R0.E->s;
+        const int one = 1;
         for (auto* R : runs)
             CCD_HIP(hipMemsetAsync(R->W->epoch, 0, (size_t)R->nTeams * R->G * sizeof(unsigned), s));
         CCD_HIP(hipMemsetAsync(W0.err, 0, sizeof(int), s));
@@ -715,13 +719,13 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
         if (launches <= 1 || budget != lastBudget) {   // the first launch starts the fits, later ones resume
             for (auto* R : runs)
                 for (auto& Q : R->teamP) { Q.resume = launches > 0 ? 1 : 0; Q.cycleBudget = budget; }
-            upload();
+            upload(s);
             lastBudget = budget;
         }
-        const auto tL = std::chrono::steady_clock::now();
         // teams that exchange need every workgroup of the grid resident at once
         const int nT = fused ? (exchanges ? 1 : total) : R0.nTeams;
         const int T = fused ? (exchanges ? total : 1) : R0.G;
+        if (evA) CCD_HIP(hipEventRecord(evA, s));
         switch (R0.m) {
         case M_LOGISTIC: launchByBP<Real, M_LOGISTIC>(R0.BP, dTeams, nT, T, s); break;
         case M_POISSON: launchByBP<Real, M_POISSON>(R0.BP, dTeams, nT, T, s); break;
@@ -729,19 +733,41 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
         default: launchByBP<Real, M_COX>(R0.BP, dTeams, nT, T, s); break;
         }
         CCD_HIP(hipGetLastError());
+        if (evB) CCD_HIP(hipEventRecord(evB, s));
         int hs[2] = {0, 0};
         CCD_HIP(hipMemcpyAsync(&hs[0], W0.err, sizeof(int), hipMemcpyDeviceToHost, s));
         CCD_HIP(hipMemcpyAsync(&hs[1], W0.allDone, sizeof(int), hipMemcpyDeviceToHost, s));
         CCD_HIP(hipStreamSynchronize(s));
-        if (hs[0]) {
-            throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
-        }
-        done = hs[1];
+        if (hs[0]) throw HipError("persistent kernel: grid exchange timed out (workgroups not co-resident?)");
         ++launches;
+        return hs[1] != 0;
+    }
+};
+
+// Launch prepared runs until all their fits have finished. Cycles per launch:
+// CCD_CYCLES_PER_LAUNCH, else adapted so a launch lasts about 4 s (the host sees
+// progress and errors at launch boundaries): 2 cycles first, then as many as the
+// measured cycle time allows, at most 32.
+template <typename Real>
+static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs, int& nLaunches) {
+    Launcher<Real> Ln(runs);
+    EngineT<Real>& E0 = *runs[0]->E;
+    hipStream_t s = E0.s;
+    int budget = 2;
+    bool fixedBudget = false;
+    if (const char* e = std::getenv("CCD_CYCLES_PER_LAUNCH")) { budget = std::max(1, std::atoi(e)); fixedBudget = true; }
+    const bool progress = std::getenv("CCD_PROGRESS") != nullptr;
+    const auto tWall = std::chrono::steady_clock::now();
+    CCD_HIP(hipEventRecord(E0.ev0, s));
+    bool done = false;
+    int cyclesRun = 0;
+    while (!done) {
+        const auto tL = std::chrono::steady_clock::now();
+        done = Ln.launch(budget);
         cyclesRun += budget;
         if (progress)
             std::fprintf(stderr, "[ccd] launch %d (%d runs, %d workgroups, %d fits): %d cycles, %s (%.1fs)\n",
-                         launches, (int)runs.size(), total, fitsAll, cyclesRun,
+                         Ln.launches, (int)runs.size(), Ln.total, Ln.fitsAll, cyclesRun,
                          done ? "all fits finished" : "fits still running",
                          std::chrono::duration<double>(std::chrono::steady_clock::now() - tWall).count());
         if (!fixedBudget && !done) {
@@ -754,7 +780,7 @@ static void launchRuns(const std::vector<ResRun<Real>*>& runs, double& deviceMs,
     float ms = 0.f;
     CCD_HIP(hipEventElapsedTime(&ms, E0.ev0, E0.ev1));
     deviceMs = ms;
-    nLaunches = launches;
+    nLaunches = Ln.launches;
 }
 
 // Results of a finished run.
@@ -840,16 +866,16 @@ void EngineT<Real>::fitResident(const ccd_fit_args& a, double* beta, ccd_fit_res
     r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Checks the batch arguments, forces the offset column of every fit to 1 and
+// returns the number of folds in row_fold.
 template <typename Real>
-void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas, ccd_fit_result* r,
-                             bool autotune) {
-    CCD_HIP(hipSetDevice(D->device));
-    const auto t0 = std::chrono::steady_clock::now();
-    const int J = D->J;
+static int checkBatch(const EngineT<Real>& E, const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas) {
+    const auto& D = *E.D;
+    const int J = D.J;
     const int n = bargs.n_fits;
     int F = 0;
     if (bargs.row_fold) {
-        for (int64_t k = 0; k < D->K; ++k) {
+        for (int64_t k = 0; k < D.K; ++k) {
             const int q = bargs.row_fold[k];
             if (q < 0 || q >= 32) throw std::invalid_argument("row_fold values must be in [0, 32)");
             F = std::max(F, q + 1);
@@ -865,59 +891,96 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
     for (int f = 0; f < n; ++f) {
         if (a.prior_type != CCD_PRIOR_NONE && !(bargs.variances[f] > 0))
             throw std::invalid_argument("variances must be > 0");
-        if (D->hasOffset) betas[(size_t)f * J] = 1.0;
+        if (D.hasOffset) betas[(size_t)f * J] = 1.0;
     }
+    return F;
+}
+
+// Team shape of a whole batch: CCD_BATCH_LAYOUT, else the fastest candidate on one
+// timed cycle of these fits (kept per batch size). False when neither applies (the
+// caller then uses chooseLayout per chunk).
+template <typename Real>
+static bool tuneLayout(EngineT<Real>& E, const ccd_fit_args& a, const ccd_batch_args& bargs, const double* betas,
+                       int F, Layout& LT) {
+    const auto& D = *E.D;
+    const int J = D.J;
+    const int n = bargs.n_fits;
+    if (!(n > 2 && !std::getenv("CCD_BATCH_LAYOUT") &&
+          !(std::getenv("CCD_AUTOTUNE") && std::atoi(std::getenv("CCD_AUTOTUNE")) == 0)))
+        return false;
+    if (!E.res) E.res = new ResWork<Real>();
+    auto& W = *static_cast<ResWork<Real>*>(E.res);
+    auto it = W.tuned.find(n);
+    if (it != W.tuned.end()) {
+        LT.Bt = it->second[0];
+        LT.T = it->second[1];
+        LT.nTeams = it->second[2];
+        return true;
+    }
+    const std::vector<Layout> cand = layoutCandidates<Real>(D.model, D.K, n, D.device);
+    if (cand.size() <= 1) return false;
+    ccd_fit_args one = a;
+    one.max_iterations = 1;
+    std::vector<double> v(n), bd(n, a.initial_bound), l(n);
+    std::vector<int32_t> h(n);
+    std::vector<int> fl(n), it2(n);
+    for (int f = 0; f < n; ++f) {
+        v[f] = bargs.variances[f];
+        h[f] = bargs.held_out_fold ? bargs.held_out_fold[f] : -1;
+    }
+    double best = 1e300;
+    for (const Layout& L : cand) {
+        std::vector<double> bb(betas, betas + (size_t)n * J);
+        double ms = 0, bytes = 0, steps = 0;
+        int nl = 0;
+        runResident<Real>(E, one, n, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(), bb.data(),
+                          fl.data(), it2.data(), l.data(), ms, bytes, steps, nl, L, bargs.fixed_index);
+        if (std::getenv("CCD_PROGRESS"))
+            std::fprintf(stderr, "[ccd] autotune %d fits: %d fits x %d workgroups per team: %.1f ms per cycle\n", n,
+                         L.Bt, L.T, ms);
+        if (ms < best) { best = ms; LT = L; }
+        if (best > 2000.0) break;   // every shape is slow here: probing costs more than it saves
+    }
+    W.tuned[n] = {LT.Bt, LT.T, LT.nTeams};
+    return true;
+}
+
+// Log prior of every fit and the result records of a finished batch.
+template <typename Real>
+static void batchResults(const EngineT<Real>& E, const ccd_fit_args& a, const double* variances, int n,
+                         const double* betas, const int* flags, const int* its, const double* lls, const int* count,
+                         double secs, ccd_fit_result* r) {
+    const int J = E.D->J;
+    for (int f = 0; f < n; ++f) {
+        double lp = 0.0;
+        for (int j = 0; j < J; ++j)
+            lp += priorLogDensity((a.exclude && a.exclude[j]) ? CCD_PRIOR_NONE : a.prior_type, variances[f],
+                                  betas[(size_t)f * J + j]);
+        r[f].return_flag = flags[f];
+        r[f].iterations = its[f];
+        r[f].find_mode_calls = count[f];
+        r[f].log_likelihood = lls[f];
+        r[f].log_prior = lp;
+        r[f].seconds = secs;
+    }
+}
+
+template <typename Real>
+void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas, ccd_fit_result* r,
+                             bool autotune) {
+    CCD_HIP(hipSetDevice(D->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int J = D->J;
+    const int n = bargs.n_fits;
+    const int F = checkBatch<Real>(*this, a, bargs, betas);
     std::vector<int> count(n, 0), flags(n, 0), its(n, 0);
     std::vector<double> lls(n, 0.0), bound(n, a.initial_bound);
     std::vector<int> pending(n);
     std::iota(pending.begin(), pending.end(), 0);
     const bool grouped = D->model == M_CLR || D->model == M_COX;
     const int64_t nGroups = D->model == M_CLR ? D->nStrata : D->K;
-    // Team shape of the whole batch: CCD_BATCH_LAYOUT, else the fastest candidate
-    // on one timed cycle of these fits (kept per batch size), else the default.
-    bool tunedOk = false;
     Layout LT;
-    if (n > 2 && autotune && !std::getenv("CCD_BATCH_LAYOUT") &&
-        !(std::getenv("CCD_AUTOTUNE") && std::atoi(std::getenv("CCD_AUTOTUNE")) == 0)) {
-        if (!res) res = new ResWork<Real>();
-        auto& W = *static_cast<ResWork<Real>*>(res);
-        auto it = W.tuned.find(n);
-        if (it != W.tuned.end()) {
-            LT.Bt = it->second[0];
-            LT.T = it->second[1];
-            LT.nTeams = it->second[2];
-            tunedOk = true;
-        } else {
-            const std::vector<Layout> cand = layoutCandidates<Real>(D->model, D->K, n, D->device);
-            if (cand.size() > 1) {
-                ccd_fit_args one = a;
-                one.max_iterations = 1;
-                std::vector<double> v(n), bd(n, a.initial_bound), l(n);
-                std::vector<int32_t> h(n);
-                std::vector<int> fl(n), it2(n);
-                for (int f = 0; f < n; ++f) {
-                    v[f] = bargs.variances[f];
-                    h[f] = bargs.held_out_fold ? bargs.held_out_fold[f] : -1;
-                }
-                double best = 1e300;
-                for (const Layout& L : cand) {
-                    std::vector<double> bb(betas, betas + (size_t)n * J);
-                    double ms = 0, bytes = 0, steps = 0;
-                    int nl = 0;
-                    runResident<Real>(*this, one, n, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(),
-                                      bb.data(), fl.data(), it2.data(), l.data(), ms, bytes, steps, nl, L,
-                                      bargs.fixed_index);
-                    if (std::getenv("CCD_PROGRESS"))
-                        std::fprintf(stderr, "[ccd] autotune %d fits: %d fits x %d workgroups per team: %.1f ms per cycle\n",
-                                     n, L.Bt, L.T, ms);
-                    if (ms < best) { best = ms; LT = L; }
-                    if (best > 2000.0) break;   // every shape is slow here: probing costs more than it saves
-                }
-                W.tuned[n] = {LT.Bt, LT.T, LT.nTeams};
-                tunedOk = true;
-            }
-        }
-    }
+    const bool tunedOk = autotune && tuneLayout<Real>(*this, a, bargs, betas, F, LT);
     while (!pending.empty()) {
         std::vector<int> retry;
         const bool whole = tunedOk && (int)pending.size() == n;
@@ -965,18 +1028,162 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
         pending.swap(retry);
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    for (int f = 0; f < n; ++f) {
-        double lp = 0.0;
-        for (int j = 0; j < J; ++j)
-            lp += priorLogDensity((a.exclude && a.exclude[j]) ? CCD_PRIOR_NONE : a.prior_type, bargs.variances[f],
-                                  betas[(size_t)f * J + j]);
-        r[f].return_flag = flags[f];
-        r[f].iterations = its[f];
-        r[f].find_mode_calls = count[f];
-        r[f].log_likelihood = lls[f];
-        r[f].log_prior = lp;
-        r[f].seconds = secs;
+    batchResults<Real>(*this, a, bargs.variances, n, betas, flags.data(), its.data(), lls.data(), count.data(), secs,
+                       r);
+}
+
+// A batch of ccd_fit_batch advanced by the caller a few cycles at a time: every fit
+// in one persistent run (one grid) whose state stays in HBM between steps. The
+// arithmetic of each cycle does not depend on where launches break, so the result
+// equals ccd_fit_batch's.
+template <typename Real>
+class SteppedBatchT : public SteppedBatch {
+public:
+    SteppedBatchT(EngineT<Real>& e, const ccd_fit_args& a, const ccd_batch_args& b, const double* betas0)
+        : E(e), n(b.n_fits), J(e.D->J) {
+        CCD_HIP(hipSetDevice(E.D->device));
+        t0 = std::chrono::steady_clock::now();
+        args = a;
+        if (a.exclude) { excl.assign(a.exclude, a.exclude + J); args.exclude = excl.data(); }
+        if (a.fixed) { fix.assign(a.fixed, a.fixed + J); args.fixed = fix.data(); }
+        args.weights = nullptr;
+        var.assign(b.variances, b.variances + n);
+        held.assign(n, -1);
+        if (b.held_out_fold) held.assign(b.held_out_fold, b.held_out_fold + n);
+        if (b.row_fold) rowFold.assign(b.row_fold, b.row_fold + E.D->K);
+        if (b.fixed_index) fixIdx.assign(b.fixed_index, b.fixed_index + n);
+        bargs = ccd_batch_args{n, var.data(), held.data(), rowFold.empty() ? nullptr : rowFold.data(),
+                               fixIdx.empty() ? nullptr : fixIdx.data()};
+        betas.assign(betas0, betas0 + (size_t)n * J);
+        const int F = checkBatch<Real>(E, args, bargs, betas.data());
+        Layout L;
+        if (!tuneLayout<Real>(E, args, bargs, betas.data(), F, L)) {
+            L = chooseLayout(E.D->K, n, E.D->device, E.D->model);
+            const bool grouped = E.D->model == M_CLR || E.D->model == M_COX;
+            const int64_t nGroups = E.D->model == M_CLR ? E.D->nStrata : E.D->K;
+            if (chunkFits(L, E.D->K, nGroups, grouped, (int)sizeof(Real)) < n)
+                throw std::invalid_argument("batch too large for one stepped run; use ccd_fit_batch");
+        }
+        lay = L;
+        std::vector<double> bd(n, args.initial_bound);
+        run = prepareResident<Real>(E, args, n, var.data(), held.data(), bargs.row_fold, F, nullptr, bd.data(),
+                                    betas.data(), L, bargs.fixed_index);
+        ptrs.push_back(&run);
+        launcher.reset(new Launcher<Real>(ptrs));
+        E.stepped = this;
     }
+    ~SteppedBatchT() override {
+        if (E.stepped == this) E.stepped = nullptr;
+    }
+
+    int step(int cycles, int32_t* doneOut) override {
+        if (cycles < 1) throw std::invalid_argument("cycles must be >= 1");
+        hipStream_t s = E.s;
+        if (!finished) {
+            finished = launcher->launch(cycles, E.timing ? E.ev0 : nullptr, E.timing ? E.ev1 : nullptr);
+            cyclesRun += cycles;
+            if (E.timing) {
+                float ms = 0.f;
+                CCD_HIP(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+                unsigned long long st = 0;
+                double b0 = 0.0;
+                CCD_HIP(hipMemcpyAsync(&st, run.W->steps, sizeof(st), hipMemcpyDeviceToHost, s));
+                CCD_HIP(hipMemcpyAsync(&b0, run.W->stats, sizeof(double), hipMemcpyDeviceToHost, s));
+                CCD_HIP(hipStreamSynchronize(s));
+                E.counters.deviceMs += ms;
+                E.counters.launches += 1;
+                E.counters.bytes += b0 - lastBytes;
+                E.counters.coordinates += (double)(st - lastSteps);
+                lastBytes = b0;
+                lastSteps = st;
+            }
+        }
+        std::vector<int> dn(n, 1);
+        if (!finished) {
+            CCD_HIP(hipMemcpyAsync(dn.data(), run.W->done, n * sizeof(int), hipMemcpyDeviceToHost, s));
+            CCD_HIP(hipStreamSynchronize(s));
+        }
+        int running = 0;
+        for (int f = 0; f < n; ++f) {
+            if (doneOut) doneOut[f] = dn[f] ? 1 : 0;
+            running += dn[f] ? 0 : 1;
+        }
+        return running;
+    }
+
+    void finish(double* betasOut, ccd_fit_result* r) override {
+        while (!finished) step(32, nullptr);
+        std::vector<int> flags(n), its(n), count(n, 1);
+        std::vector<double> lls(n);
+        double algo = 0.0, steps = 0.0;
+        gatherRun<Real>(run, betas.data(), flags.data(), its.data(), lls.data(), algo, steps);
+        E.stepped = nullptr;   // the engine's workspace is free again
+        // update()'s retry of ill-conditioned fits with a tenth of the bound
+        // (CyclicCoordinateDescent.cpp:599-614), as ccd_fit_batch does
+        std::vector<int> retry;
+        for (int f = 0; f < n; ++f)
+            if (flags[f] == CCD_ILLCONDITIONED && args.max_bound_count > 1) retry.push_back(f);
+        if (!retry.empty()) {
+            const int m = (int)retry.size();
+            std::vector<double> v(m), bb((size_t)m * J, 0.0);
+            std::vector<int32_t> h(m), fj(m);
+            for (int i = 0; i < m; ++i) {
+                v[i] = var[retry[i]];
+                h[i] = held[retry[i]];
+                fj[i] = fixIdx.empty() ? -1 : fixIdx[retry[i]];
+            }
+            ccd_fit_args a2 = args;
+            a2.initial_bound = args.initial_bound / 10.0;
+            a2.max_bound_count = args.max_bound_count - 1;
+            ccd_batch_args b2{m, v.data(), h.data(), bargs.row_fold, fixIdx.empty() ? nullptr : fj.data()};
+            std::vector<ccd_fit_result> r2(m);
+            E.fitBatch(a2, b2, bb.data(), r2.data(), false);
+            for (int i = 0; i < m; ++i) {
+                const int f = retry[i];
+                std::copy(bb.begin() + (size_t)i * J, bb.begin() + (size_t)(i + 1) * J, betas.begin() + (size_t)f * J);
+                flags[f] = r2[i].return_flag;
+                its[f] = r2[i].iterations;
+                lls[f] = r2[i].log_likelihood;
+                count[f] = 1 + r2[i].find_mode_calls;
+            }
+        }
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::copy(betas.begin(), betas.end(), betasOut);
+        batchResults<Real>(E, args, var.data(), n, betas.data(), flags.data(), its.data(), lls.data(), count.data(),
+                           secs, r);
+    }
+
+    int fits() const override { return n; }
+    int cycles() const override { return cyclesRun; }
+    void layout(int32_t* fitsPerTeam, int32_t* workgroupsPerTeam, int32_t* teams) const override {
+        *fitsPerTeam = lay.Bt;
+        *workgroupsPerTeam = lay.T;
+        *teams = lay.nTeams;
+    }
+
+private:
+    EngineT<Real>& E;
+    int n, J;
+    int cyclesRun = 0;
+    bool finished = false;
+    ccd_fit_args args{};
+    ccd_batch_args bargs{};
+    std::vector<unsigned char> excl, fix;
+    std::vector<double> var, betas;
+    std::vector<int32_t> held, rowFold, fixIdx;
+    Layout lay;
+    ResRun<Real> run;
+    std::vector<ResRun<Real>*> ptrs;
+    std::unique_ptr<Launcher<Real>> launcher;
+    double lastBytes = 0.0;
+    unsigned long long lastSteps = 0;
+    std::chrono::steady_clock::time_point t0;
+};
+
+template <typename Real>
+SteppedBatch* EngineT<Real>::beginStepped(const ccd_fit_args& a, const ccd_batch_args& b, const double* betas0) {
+    if (stepped) throw std::runtime_error("a stepped batch is already open on this engine");
+    return new SteppedBatchT<Real>(*this, a, b, betas0);
 }
 
 // Single fits of several datasets as one persistent grid (CyclicCoordinateDescent::

@@ -336,7 +336,7 @@ int ccd_server_submit(ccd_server_handle s, const ccd_data_desc* d, const ccd_fit
                 if ((j == 0 && d->has_offset) || r->beta[j] != 0.0) ++live;
             }
             if (live > 32)
-                bytes += (double)(K + 1) * 8.0 + (double)idxNnz * (4.0 + (d->precision == CCD_FP32 ? 4.0 : 8.0));
+                bytes += (double)(K + 1) * 8.0 + (double)idxNnz * (4.0 + (ccd::precisionOf(d->precision) == CCD_FP32 ? 4.0 : 8.0));
         }
         r->footprint = bytes;
         r->blockSize = blockBytes(bytes);

@@ -76,6 +76,11 @@ def lib():
         L.ccd_get_kernel_stats.argtypes = [vp, C.POINTER(_abi.KernelStats)]
         L.ccd_reset_kernel_stats.argtypes = [vp]
         L.ccd_estimate_footprint.argtypes = [C.POINTER(_abi.DataDesc), dp]
+        L.ccd_copy_bandwidth.argtypes = [C.c_int, C.c_double, i32, dp]
+        L.ccd_batch_begin.argtypes = [vp, C.POINTER(_abi.FitArgs), C.POINTER(_abi.BatchArgs), dp, C.POINTER(vp)]
+        L.ccd_batch_step.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(i32)]
+        L.ccd_batch_end.argtypes = [vp, dp, C.POINTER(_abi.FitResult)]
+        L.ccd_batch_layout.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
         if hasattr(L, "ccd_server_start"):
             L.ccd_server_start.argtypes = [C.POINTER(_abi.ServerConfig), C.POINTER(vp)]
             L.ccd_server_submit.argtypes = [vp, C.POINTER(_abi.DataDesc), C.POINTER(_abi.FitArgs), dp,
@@ -102,16 +107,23 @@ def _d(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def copy_bandwidth(device=0, nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth of the device (GB/s, read + write) from a copy kernel."""
+    v = C.c_double()
+    check(lib().ccd_copy_bandwidth(int(device), float(nbytes), int(reps), C.byref(v)))
+    return v.value
+
+
 class Engine:
     """Device engine for one dataset: the AbstractModelSpecifics interface."""
 
     def __init__(self, data, precision=_abi.FP64, device=0, _handle=None):
         self.data = data
-        self.precision = precision
+        self.precision = _abi.precision_of(precision)
         if _handle is not None:
             self.h = _handle
             return
-        self._desc = data.desc(precision)
+        self._desc = data.desc(self.precision)
         h = C.c_void_p()
         check(lib().ccd_create(C.byref(self._desc), device, C.byref(h)))
         self.h = h
@@ -216,27 +228,41 @@ class Engine:
         check(lib().ccd_fit(self.h, C.byref(hold.args), m, _d(beta), C.byref(res)))
         return beta, res.as_dict()
 
-    def fit_batch(self, variances, prior_type="laplace", control=None, held_out_fold=None, row_fold=None,
-                  betas0=None, exclude=(), fixed_index=None):
+    def _batch_args(self, variances, prior_type, control, held_out_fold, row_fold, betas0, exclude, fixed_index):
         control = control or create_control()
         n = len(variances)
         prior = create_prior(prior_type, float(variances[0]), exclude=exclude)
-        hold = FitArgsHolder(self.data.J, prior, control, intercept_index=intercept_index(self.data))
-        var = np.ascontiguousarray(variances, dtype=np.float64)
-        hof = None if held_out_fold is None else np.ascontiguousarray(held_out_fold, dtype=np.int32)
-        rf = None if row_fold is None else np.ascontiguousarray(row_fold, dtype=np.int32)
+        keep = {"hold": FitArgsHolder(self.data.J, prior, control, intercept_index=intercept_index(self.data)),
+                "var": np.ascontiguousarray(variances, dtype=np.float64),
+                "hof": None if held_out_fold is None else np.ascontiguousarray(held_out_fold, dtype=np.int32),
+                "rf": None if row_fold is None else np.ascontiguousarray(row_fold, dtype=np.int32),
+                "fi": None if fixed_index is None else np.ascontiguousarray(fixed_index, dtype=np.int32)}
         b = _abi.BatchArgs()
         b.n_fits = n
-        b.variances = _d(var)
-        b.held_out_fold = _abi.ptr(hof, C.c_int32)
-        b.row_fold = _abi.ptr(rf, C.c_int32)
-        fi = None if fixed_index is None else np.ascontiguousarray(fixed_index, dtype=np.int32)
-        b.fixed_index = _abi.ptr(fi, C.c_int32)
+        b.variances = _d(keep["var"])
+        b.held_out_fold = _abi.ptr(keep["hof"], C.c_int32)
+        b.row_fold = _abi.ptr(keep["rf"], C.c_int32)
+        b.fixed_index = _abi.ptr(keep["fi"], C.c_int32)
         betas = np.zeros((n, self.data.J)) if betas0 is None else np.array(betas0, dtype=np.float64)
-        betas = np.ascontiguousarray(betas)
+        return b, np.ascontiguousarray(betas), keep
+
+    def fit_batch(self, variances, prior_type="laplace", control=None, held_out_fold=None, row_fold=None,
+                  betas0=None, exclude=(), fixed_index=None):
+        n = len(variances)
+        b, betas, keep = self._batch_args(variances, prior_type, control, held_out_fold, row_fold, betas0, exclude,
+                                          fixed_index)
         results = (_abi.FitResult * n)()
-        check(lib().ccd_fit_batch(self.h, C.byref(hold.args), C.byref(b), _d(betas), results))
+        check(lib().ccd_fit_batch(self.h, C.byref(keep["hold"].args), C.byref(b), _d(betas), results))
         return betas, [r.as_dict() for r in results]
+
+    def begin_batch(self, variances, prior_type="laplace", control=None, held_out_fold=None, row_fold=None,
+                    betas0=None, exclude=(), fixed_index=None):
+        """The fits of fit_batch, advanced by SteppedBatch.step(cycles) (ccd_batch_begin)."""
+        b, betas, keep = self._batch_args(variances, prior_type, control, held_out_fold, row_fold, betas0, exclude,
+                                          fixed_index)
+        h = C.c_void_p()
+        check(lib().ccd_batch_begin(self.h, C.byref(keep["hold"].args), C.byref(b), _d(betas), C.byref(h)))
+        return SteppedBatch(self, h, len(variances))
 
     def cross_validate(self, prior, control, selector=None, refit_fp64=True):
         """Grid-search CV (fold x grid fits batched on this engine) and the final fit at
@@ -323,6 +349,47 @@ class Engine:
 
     def reset_kernel_stats(self):
         check(lib().ccd_reset_kernel_stats(self.h))
+
+
+class SteppedBatch:
+    """A batch of fits advanced a few CCD cycles at a time (ccd_batch_step); end()
+    returns what fit_batch would have returned for the same fits."""
+
+    def __init__(self, engine, handle, n):
+        self.engine = engine
+        self.h = handle
+        self.n = n
+
+    def step(self, cycles=1):
+        """Runs up to `cycles` more cycles; returns (fits still running, done flags [n])."""
+        done = np.zeros(self.n, dtype=np.int32)
+        running = C.c_int32()
+        check(lib().ccd_batch_step(self.h, int(cycles), done.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   C.byref(running)))
+        return running.value, done.astype(bool)
+
+    def layout(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().ccd_batch_layout(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"fits_per_team": a.value, "workgroups_per_team": b.value, "teams": c.value}
+
+    def end(self):
+        betas = np.zeros((self.n, self.engine.data.J))
+        results = (_abi.FitResult * self.n)()
+        h, self.h = self.h, None
+        check(lib().ccd_batch_end(h, _d(betas), results))
+        return betas, [r.as_dict() for r in results]
+
+    def abandon(self):
+        if self.h:
+            h, self.h = self.h, None
+            check(lib().ccd_batch_end(h, None, None))
+
+    def __del__(self):
+        try:
+            self.abandon()
+        except Exception:
+            pass
 
 
 def fit_cyclops_model(data, prior=None, control=None, weights=None, fixed=None, beta0=None,

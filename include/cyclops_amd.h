@@ -64,9 +64,13 @@ extern "C" {
 #define CCD_MAX_ITERATIONS  2
 #define CCD_ILLCONDITIONED  3
 
-/* Precision of the bulk arithmetic (PrecisionType, src/cyclops/Types.h:155) */
-#define CCD_FP64  64
-#define CCD_FP32  32
+/* Precision of the bulk arithmetic: PrecisionType (src/cyclops/Types.h:155-160,
+ * FP64 = 0, FP32 = 1). The bit counts 64 and 32 are accepted as aliases; FP16 (2)
+ * is refused with CCD_ERR_INVALID. */
+#define CCD_FP64       0
+#define CCD_FP32       1
+#define CCD_FP64_BITS  64
+#define CCD_FP32_BITS  32
 
 /* ---- data description ---------------------------------------------------
  * Column store in the reference's layout (CompressedDataColumn,
@@ -87,7 +91,7 @@ extern "C" {
  */
 typedef struct ccd_data_desc {
     int32_t        model_type;
-    int32_t        precision;       /* CCD_FP64 or CCD_FP32 */
+    int32_t        precision;       /* CCD_FP64 / CCD_FP32 (or 64 / 32) */
     int64_t        K;               /* rows */
     int32_t        J;               /* columns */
     int32_t        has_offset;
@@ -214,6 +218,32 @@ typedef struct ccd_batch_args {
 } ccd_batch_args;
 int ccd_fit_batch(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args* batch,
                   double* betas, ccd_fit_result* results);
+
+/* ---- stepped batched fits --------------------------------------------------
+ * The fits of ccd_fit_batch (same arguments, same results) advanced by the caller
+ * a given number of CCD cycles per call, so that a caller can watch fits finish,
+ * interleave its own work between cycles, or time single cycles. The cycle loop is
+ * findMode's (CyclicCoordinateDescent.cpp:1008-1021, 1202-1207); no reference entry
+ * point steps it from outside. All fits share one persistent run whose state stays
+ * in HBM between steps; where the steps break changes no result.
+ *   ccd_batch_begin : set-up (copies every argument; betas0 [n_fits][J] are the
+ *                     starting values). While the batch is open, the engine runs no
+ *                     other whole fit (CCD_ERR_STATE) and cannot be destroyed.
+ *   ccd_batch_step  : up to `cycles` more cycles of every unfinished fit. done[f]
+ *                     (optional, [n_fits]) becomes 1 once fit f has converged or hit
+ *                     max_iterations; *running receives the fits still running.
+ *   ccd_batch_end   : runs what remains, applies update()'s bound retries
+ *                     (CyclicCoordinateDescent.cpp:599-614) and returns betas
+ *                     [n_fits][J] and results, then frees the batch. betas and
+ *                     results may both be NULL to abandon the batch.
+ *   ccd_batch_layout: the team shape the batch runs with (fits per team, workgroups
+ *                     per team, teams). */
+typedef struct ccd_batch_run* ccd_batch_handle;
+int ccd_batch_begin(ccd_handle h, const ccd_fit_args* common, const ccd_batch_args* batch,
+                    const double* betas0, ccd_batch_handle* out);
+int ccd_batch_step(ccd_batch_handle b, int32_t cycles, int32_t* done, int32_t* running);
+int ccd_batch_end(ccd_batch_handle b, double* betas, ccd_fit_result* results);
+int ccd_batch_layout(ccd_batch_handle b, int32_t* fits_per_team, int32_t* workgroups_per_team, int32_t* teams);
 
 /* ---- fits of several datasets in one launch ---------------------------------
  * One fit per handle (its own data, args[i] and starting betas[i] [J_i], overwritten
@@ -346,6 +376,12 @@ typedef struct ccd_kernel_stats {
 } ccd_kernel_stats;
 int ccd_get_kernel_stats(ccd_handle h, ccd_kernel_stats* stats);
 int ccd_reset_kernel_stats(ccd_handle h);
+/* Achievable HBM bandwidth of `device`: a copy kernel over two buffers of `bytes`
+ * each (16-byte loads and stores), timed with HIP events over `reps` launches after
+ * one untimed launch; *gbs = 2 * bytes * reps / seconds / 1e9 (read + write). No
+ * reference counterpart: a measurement hook that puts kernel figures beside the
+ * bandwidth the device delivers. */
+int ccd_copy_bandwidth(int device, double bytes, int32_t reps, double* gbs);
 
 #ifdef __cplusplus
 }

@@ -81,3 +81,42 @@ def test_from_matrices_assembles_like_the_reference():
     assert np.array_equal(a.rows, b.rows) and np.allclose(a.values, b.values)
     assert np.array_equal(a.dense_values, b.dense_values) and a.has_offset and b.has_offset
     assert np.array_equal(a.to_dense(), b.to_dense())
+
+
+def test_precision_takes_the_reference_enum_values():
+    """PrecisionType (src/cyclops/Types.h:155-160): FP64 = 0 and FP32 = 1 pass through
+    unchanged, the bit counts 64 / 32 are aliases of them, and FP16 (2) is refused.
+    ccd_estimate_footprint reads the precision without touching a device."""
+    import ctypes as Cc
+    import _cases
+    from cyclops_amd import _abi, engine
+    assert (_abi.FP64, _abi.FP32) == (0, 1)
+    d = _cases.logistic()
+    L = engine.lib()
+
+    def footprint(p):
+        desc = d.desc(p)
+        out = Cc.c_double()
+        rc = L.ccd_estimate_footprint(Cc.byref(desc), Cc.byref(out))
+        return rc, out.value
+
+    assert footprint(0)[0] == 0 and footprint(1)[0] == 0
+    assert footprint(0) == footprint(64)
+    assert footprint(1) == footprint(32)
+    assert footprint(0)[1] > footprint(1)[1]   # FP64 state takes more room
+    rc, _ = footprint(2)
+    assert rc == _abi.ERR_INVALID
+    assert "precision" in L.ccd_last_error().decode()
+    assert _abi.precision_of(64) == _abi.FP64 and _abi.precision_of(_abi.FP32) == _abi.FP32
+    with pytest.raises(ValueError):
+        _abi.precision_of(2)
+
+
+def test_stepped_batch_refuses_bad_handles_without_a_device():
+    """The stepped-batch entry points check their handles before touching a device."""
+    import ctypes as Cc
+    from cyclops_amd import _abi, engine
+    L = engine.lib()
+    running = Cc.c_int32()
+    assert L.ccd_batch_step(None, 1, None, Cc.byref(running)) == _abi.ERR_INVALID
+    assert L.ccd_batch_end(None, None, None) == _abi.ERR_INVALID
