@@ -768,7 +768,7 @@ ResidentNeeds residentNeeds(const ccd_data_desc& d, int device) {
     n.capacityFraction = (double)G / std::max(1, occ * cus);
     const double s = precisionOf(d.precision) == CCD_FP32 ? 4.0 : 8.0;
     const bool grouped = d.model_type == CCD_MODEL_CONDITIONAL_LOGISTIC || d.model_type == CCD_MODEL_COX;
-    n.workspaceBytes = d.K * (s + 12.0) + (grouped ? (d.K + 1) * 5.0 * 8.0 : 0.0) +
+    n.workspaceBytes = d.K * (s + (s == 4.0 ? 2.0 : 0.0) + 12.0) + (grouped ? (d.K + 1) * 5.0 * 8.0 : 0.0) +
                        (double)d.J * ((G + 1) * 4.0 + G * 16.0 + 32.0 * 8.0 + 64.0) + (1 << 20);
     return n;
 }

@@ -35,6 +35,21 @@ __device__ __forceinline__ Real xOf(const Col<Real>& c, int64_t e) {
 template <int FMT>
 constexpr bool isIndicatorFmt() { return FMT == F_INDICATOR || FMT == F_INTERCEPT; }
 
+// bfloat16 bits of a float, rounded to nearest even (finite inputs), and back: the
+// storage of the low part of a compensated FP32 xb.
+__host__ __device__ __forceinline__ unsigned short bf16Bits(float x) {
+    unsigned u;
+    __builtin_memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16Value(unsigned short b) {
+    const unsigned u = (unsigned)b << 16;
+    float x;
+    __builtin_memcpy(&x, &u, 4);
+    return x;
+}
+
 __device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 

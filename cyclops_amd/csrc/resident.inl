@@ -108,23 +108,27 @@ __global__ __launch_bounds__(BLOCK) void kXBetaRows(int64_t K, int B, int Bt, in
                                                    const int64_t* __restrict__ rowPtr,
                                                    const int32_t* __restrict__ rowCol,
                                                    const Real* __restrict__ rowVal,
-                                                   const double* __restrict__ beta, Real* __restrict__ xb) {
+                                                   const double* __restrict__ beta, Real* __restrict__ xb,
+                                                   unsigned short* __restrict__ xbLo) {
     const int64_t total = K * (int64_t)B;
     const int64_t stride = (int64_t)gridDim.x * BLOCK;
     for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < total; t += stride) {
         const int64_t k = t / B;
         const int f = (int)(t % B);
         const double* bf = beta + (size_t)f * J;
-        Real acc = Real(0);
-        for (int64_t e = rowPtr[k]; e < rowPtr[k + 1]; ++e) acc += (Real)bf[rowCol[e]] * rowVal[e];
-        xb[(size_t)(f / Bt) * K * BP + k * BP + f % Bt] = acc;
+        double acc = 0.0;
+        for (int64_t e = rowPtr[k]; e < rowPtr[k + 1]; ++e) acc += bf[rowCol[e]] * (double)rowVal[e];
+        const size_t q = (size_t)(f / Bt) * K * BP + k * BP + f % Bt;
+        xb[q] = (Real)acc;
+        if (xbLo) xbLo[q] = bf16Bits((float)(acc - (double)xb[q]));   // FP32 runs: the rounding remainder
     }
 }
 
 // xb[k*BP + f] += beta_f * x_kj for every fit f (one column).
 template <typename Real, int FMT>
 __global__ __launch_bounds__(BLOCK) void kAxpyBatch(Col<Real> c, const double* __restrict__ beta, int J, int j,
-                                                   int B, int Bt, int BP, int64_t K, Real* __restrict__ xb) {
+                                                   int B, int Bt, int BP, int64_t K, Real* __restrict__ xb,
+                                                   unsigned short* __restrict__ xbLo) {
     // xb of fit f lives in team f / Bt, lane f % Bt: xb[(f/Bt)*K*BP + row*BP + f%Bt]
     const int64_t total = c.n * (int64_t)B;
     const int64_t stride = (int64_t)gridDim.x * BLOCK;
@@ -133,7 +137,14 @@ __global__ __launch_bounds__(BLOCK) void kAxpyBatch(Col<Real> c, const double* _
         const int f = (int)(t % B);
         const double bf = beta[(size_t)f * J + j];
         if (bf == 0.0) continue;
-        xb[(size_t)(f / Bt) * K * BP + rowOf<Real, FMT>(c, e) * BP + f % Bt] += (Real)bf * xOf<Real, FMT>(c, e);
+        const size_t q = (size_t)(f / Bt) * K * BP + rowOf<Real, FMT>(c, e) * BP + f % Bt;
+        if (xbLo) {   // FP32 runs: compensated, as the kernel's own updates
+            const double sum = (double)xb[q] + (double)bf16Value(xbLo[q]) + bf * (double)xOf<Real, FMT>(c, e);
+            xb[q] = (Real)sum;
+            xbLo[q] = bf16Bits((float)(sum - (double)xb[q]));
+        } else {
+            xb[q] += (Real)bf * xOf<Real, FMT>(c, e);
+        }
     }
 }
 
@@ -160,6 +171,7 @@ struct ResWork {
     int* pseg = nullptr;
     // per batch
     Real* xb = nullptr;
+    unsigned short* xbLo = nullptr;   // FP32 runs: low part of xb (bfloat16 bits)
     double *denom = nullptr, *Dacc = nullptr, *R1 = nullptr, *R2 = nullptr, *nW = nullptr;
     double *beta = nullptr, *hDelta = nullptr, *var = nullptr, *initBound = nullptr;
     unsigned char *exclude = nullptr, *fixed = nullptr;
@@ -183,7 +195,7 @@ struct ResWork {
     std::map<int, std::array<int, 3>> tuned;   // batch size -> {Bt, T, nTeams} picked by timing
 
     void freeBatch() {
-        dfree(xb); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
+        dfree(xb); dfree(xbLo); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
@@ -455,6 +467,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
         W.nTeams = nTeams;
         double& Bb = W.bytes;
         W.xb = dalloc<Real>((size_t)nTeams * K * BP, Bb);
+        if (sizeof(Real) == 4) W.xbLo = dalloc<unsigned short>((size_t)nTeams * K * BP, Bb);
         if (grouped) {
             const size_t pn = (size_t)nTeams * N * BP;
             W.denom = dalloc<double>(pn, Bb);
@@ -563,6 +576,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     // xb = X beta for every fit (column order, as CyclicCoordinateDescent::computeXBeta);
     // team t's block is xb[t*K*BP ...], fit-minor inside the team
     CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)nTeams * K * BP * sizeof(Real), s));
+    if (W.xbLo) CCD_HIP(hipMemsetAsync(W.xbLo, 0, (size_t)nTeams * K * BP * sizeof(unsigned short), s));
     {
         // warm starts with many non-zero sparse columns form the sparse part of xb in
         // one pass over rows; otherwise (and for DENSE/INTERCEPT columns) one axpy per column
@@ -576,14 +590,14 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
         if (byRows) {
             D.buildRowMajor(s);
             kXBetaRows<Real><<<gridFor(K * (int64_t)B), BLOCK, 0, s>>>(K, B, L.Bt, BP, J, D.dRowPtr, D.dRowCol,
-                                                                      D.dRowVal, W.beta, W.xb);
+                                                                      D.dRowVal, W.beta, W.xb, W.xbLo);
             CCD_HIP(hipGetLastError());
         }
         for (int j = 0; j < J; ++j) {
             const Col<Real> c = E.col(j);
             if (!live[j] || c.n == 0) continue;
             if (byRows && (D.fmt[j] == F_INDICATOR || D.fmt[j] == F_SPARSE)) continue;
-            CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, W.beta, J, j, B, L.Bt, BP, K, W.xb)));
+            CCD_DISPATCH_FMT(D.fmt[j], (kAxpyBatch<Real, FMT><<<gridFor(c.n * (int64_t)B), BLOCK, 0, s>>>(c, W.beta, J, j, B, L.Bt, BP, K, W.xb, W.xbLo)));
         }
     }
     CCD_HIP(hipGetLastError());
@@ -624,6 +638,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
         const size_t f0 = (size_t)t * L.Bt;
         Q.B = std::min(L.Bt, B - (int)f0);
         Q.xb = W.xb + (size_t)t * K * BP;
+        Q.xbLo = W.xbLo ? W.xbLo + (size_t)t * K * BP : nullptr;
         if (grouped) {
             const size_t o = (size_t)t * N * BP;
             Q.denom = W.denom + o; Q.Dacc = W.Dacc + o; Q.R1 = W.R1 + o; Q.R2 = W.R2 + o; Q.nW = W.nW + o;

@@ -69,6 +69,7 @@ struct Params {
     const int* fixJ;      // [B] column held at its starting value in fit f (-1: none), or null
     // fit state
     Real* xb;             // [K*BP]
+    unsigned short* xbLo; // [K*BP] FP32 runs: low part of xb as bfloat16 bits (xb + low carries the sum), else null
     double* denom;        // [N*BP] (CLR/Cox)
     double* Dacc;         // [N*BP] (Cox)
     double* R1;           // [N*BP]
@@ -185,6 +186,8 @@ struct Geo {
     static constexpr int SUB = BP < 64 ? BP : 64;   // lanes per entry
     static constexpr int NSG = BLOCK / SUB;         // sub-groups per workgroup
     static constexpr int FPL = BP / SUB;            // fits per lane
+    // fit i of lane sl of a sub-group: a lane holds FPL adjacent fits
+    __host__ __device__ static constexpr int fit(int sl, int i) { return sl * FPL + i; }
 };
 
 template <typename Real, int FMT>
@@ -250,6 +253,14 @@ struct GeoM {
     static constexpr int SUB = WIDE ? (BP / 4 < 64 ? BP / 4 : 64) : Geo<BP>::SUB;
     static constexpr int NSG = BLOCK / SUB;
     static constexpr int FPL = BP / SUB;
+    // a lane holds FPL adjacent fits, so one entry's state is one vector load per lane
+    __host__ __device__ static constexpr int fit(int sl, int i) { return sl * FPL + i; }
+};
+
+// FPL adjacent per-fit values of one row (fit-minor state): one vector access.
+template <typename T, int N>
+struct alignas(sizeof(T) * N) VecN {
+    T v[N];
 };
 template <int BP, int MODEL>
 struct UnrollM {
@@ -265,6 +276,7 @@ struct Batch {
     int kk[U];
     Real xx[U];
     Real xv[U][GE::FPL];
+    Real xl[U][GE::FPL];   // low parts of xv (compensated FP32 runs, fast path)
     Real ww[U][GE::FPL];
     int sgm[U], pp[U];
     double r1[U][GE::FPL], r2[U][GE::FPL];
@@ -288,20 +300,44 @@ __device__ __forceinline__ void loadRows(const Params<Real>& P, int j, int64_t e
     }
 }
 
-// Per-fit state of the rows of a batch (xb, weights, risk-set ratios).
-template <typename Real, int MODEL, int BP, int U>
+// Per-fit state of the rows of a batch (xb, weights, risk-set ratios); LO also
+// fetches the low parts of xb for an update from registers (they are requested with
+// the gathers, so they cost no latency of their own).
+template <typename Real, int MODEL, int BP, int U, bool LO = false>
 __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on)[GeoM<BP, MODEL>::FPL],
                                           Batch<Real, MODEL, BP, U>& bt) {
     using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sl = threadIdx.x % GE::SUB;
+    constexpr int FPL = GE::FPL;
+    if constexpr (FPL > 1) {
+        // the lane's adjacent fits in one access each; values of fits that are not
+        // active are kept as loaded (an update from registers writes them back as they are)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t q = (int64_t)bt.kk[u] * BP + GE::fit(sl, 0);
+            VecN<Real, FPL> v{};
+            if (bt.val[u]) v = *reinterpret_cast<const VecN<Real, FPL>*>(P.xb + q);
+#pragma unroll
+            for (int i = 0; i < FPL; ++i) bt.xv[u][i] = v.v[i];
+            if constexpr (LO) {
+                VecN<unsigned short, FPL> lo{};
+                if (bt.val[u]) lo = *reinterpret_cast<const VecN<unsigned short, FPL>*>(P.xbLo + q);
+#pragma unroll
+                for (int i = 0; i < FPL; ++i) bt.xl[u][i] = (Real)bf16Value(lo.v[i]);
+            }
+        }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int i = 0; i < GE::FPL; ++i) {
-            const int f = sl + i * GE::SUB;
+            const int f = GE::fit(sl, i);
             const bool go = bt.val[u] && on[i];
-            bt.xv[u][i] = go ? P.xb[(int64_t)bt.kk[u] * BP + f] : Real(0);
+            if constexpr (FPL == 1) {
+                bt.xv[u][i] = go ? P.xb[(int64_t)bt.kk[u] * BP + f] : Real(0);
+                if constexpr (LO) bt.xl[u][i] = go ? (Real)bf16Value(P.xbLo[(int64_t)bt.kk[u] * BP + f]) : Real(0);
+            }
             if constexpr (MODEL != M_CLR) bt.ww[u][i] = go ? weightOf(P, bt.kk[u], f) : Real(0);
             if constexpr (GROUPED) {
                 const size_t q = (size_t)bt.pp[u] * BP + f;
@@ -311,11 +347,29 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
         }
 }
 
-template <typename Real, int MODEL, int BP, int FMT, int U>
+template <typename Real, int MODEL, int BP, int FMT, int U, bool LO = false>
 __device__ __forceinline__ void loadBatch(const Params<Real>& P, int j, int64_t e, int64_t n1,
                                           const bool (&on)[GeoM<BP, MODEL>::FPL], Batch<Real, MODEL, BP, U>& bt) {
     loadRows<Real, MODEL, BP, FMT, U>(P, j, e, n1, bt);
-    loadState<Real, MODEL, BP, U>(P, on, bt);
+    loadState<Real, MODEL, BP, U, LO>(P, on, bt);
+}
+
+// xb + delta * x. FP32 runs keep xb as an FP32 value plus a low part (the rounding
+// remainder, stored as a bfloat16), add in FP64 and split again, so no update is
+// lost to rounding: an FP32 xb alone loses every step under half an ulp of xb and
+// drifts from X beta over hundreds of cycles, which moved FP32 coefficients by up to
+// 1e-3 relative against FP64 on the 1e6-row Poisson workload; with the low part they
+// stay within 3e-5. The column pass reads the FP32 value only. FP64 runs add in
+// place, as the reference does.
+template <typename Real>
+__device__ __forceinline__ void addXb(Real& hi, Real& lo, double delta, Real x) {
+    if constexpr (sizeof(Real) == 4) {
+        const double sum = (double)hi + (double)lo + delta * (double)x;
+        hi = (Real)sum;
+        lo = (Real)(sum - (double)hi);
+    } else {
+        hi = hi + (Real)delta * x;
+    }
 }
 
 // Gradient terms of a batch, strictly in entry order.
@@ -408,29 +462,65 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n
 
 // Update of a chunk that fitted in one batch: xb from the registers of the column
 // pass (fits that moved were active there), per-run denominators as updatePass.
-template <typename Real, int MODEL, int BP, int U>
+// LOW: the low parts of xb (FP32 runs) came with the registers; otherwise they are
+// read here. A lane holding several fits writes them back in one store per entry,
+// the fits that did not move with the values they had.
+template <typename Real, int MODEL, int BP, int U, bool LOW>
 __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Real, MODEL, BP, U>& bt,
                                            const double (&dl)[GeoM<BP, MODEL>::FPL]) {
     using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    constexpr bool COMP = sizeof(Real) == 4;   // compensated xb
+    constexpr int FPL = GE::FPL;
     const int sl = threadIdx.x % GE::SUB;
-    double acc[GE::FPL];
+    double acc[FPL];
+    bool anyLane = false;
 #pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
+    for (int i = 0; i < FPL; ++i) {
+        acc[i] = 0.0;
+        anyLane = anyLane || dl[i] != 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         if (!bt.val[u]) break;
+        const int64_t q0 = (int64_t)bt.kk[u] * BP + GE::fit(sl, 0);
+        VecN<Real, FPL> nvv;
+        VecN<unsigned short, FPL> nlv{};
+        if constexpr (COMP) {
+            if constexpr (LOW) {
 #pragma unroll
-        for (int i = 0; i < GE::FPL; ++i) {
+                for (int i = 0; i < FPL; ++i) nlv.v[i] = bf16Bits((float)bt.xl[u][i]);
+            } else if constexpr (FPL > 1) {
+                if (anyLane) nlv = *reinterpret_cast<const VecN<unsigned short, FPL>*>(P.xbLo + q0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < FPL; ++i) {
+            nvv.v[i] = bt.xv[u][i];
             if (dl[i] == 0.0) continue;
-            const int f = sl + i * GE::SUB;
             const Real old = bt.xv[u][i];
-            const Real nv = old + (Real)dl[i] * bt.xx[u];
-            P.xb[(int64_t)bt.kk[u] * BP + f] = nv;
+            Real nv = old, nl = Real(0);
+            if constexpr (COMP) {
+                if constexpr (!LOW && FPL == 1) nlv.v[i] = P.xbLo[q0 + i];
+                nl = (Real)bf16Value(nlv.v[i]);
+            }
+            addXb<Real>(nv, nl, dl[i], bt.xx[u]);
+            nvv.v[i] = nv;
+            if constexpr (COMP) nlv.v[i] = bf16Bits((float)nl);
+            if constexpr (FPL == 1) {
+                P.xb[q0 + i] = nv;
+                if constexpr (COMP) P.xbLo[q0 + i] = nlv.v[i];
+            }
             if constexpr (GROUPED) {
                 Real diff = dexp(nv) - dexp(old);
                 if constexpr (MODEL == M_COX) diff *= bt.ww[u][i];
                 acc[i] += (double)diff;
+            }
+        }
+        if constexpr (FPL > 1) {
+            if (anyLane) {
+                *reinterpret_cast<VecN<Real, FPL>*>(P.xb + q0) = nvv;
+                if constexpr (COMP) *reinterpret_cast<VecN<unsigned short, FPL>*>(P.xbLo + q0) = nlv;
             }
         }
         if constexpr (GROUPED) {
@@ -438,9 +528,9 @@ __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Re
             if (pn != bt.pp[u]) {   // last entry of a run of equal pid
                 const int p = bt.pp[u];
 #pragma unroll
-                for (int i = 0; i < GE::FPL; ++i) {
+                for (int i = 0; i < FPL; ++i) {
                     if (dl[i] == 0.0) continue;
-                    const int f = sl + i * GE::SUB;
+                    const int f = GE::fit(sl, i);
                     const size_t q = (size_t)p * BP + f;
                     const double dn = P.denom[q] + acc[i];
                     P.denom[q] = dn;
@@ -463,13 +553,18 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
     using GE = GeoM<BP, MODEL>;
     constexpr int U = UnrollM<BP, MODEL>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    constexpr int FPL = GE::FPL;
     const int sl = threadIdx.x % GE::SUB;
     double acc[GE::FPL];
+    bool anyLane = false;   // some fit of this lane moved
 #pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
+    for (int i = 0; i < GE::FPL; ++i) {
+        acc[i] = 0.0;
+        anyLane = anyLane || dl[i] != 0.0;
+    }
     for (int64_t e = n0; e < n1; e += U) {
         int64_t kk[U];
-        Real xx[U], xv[U][GE::FPL], ww[U][GE::FPL];
+        Real xx[U], xv[U][GE::FPL], xl[U][GE::FPL], ww[U][GE::FPL];
         int pp[U], pn[U];
         bool val[U];
 #pragma unroll
@@ -482,29 +577,70 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
                 pn[u] = (val[u] && e + u + 1 < n1) ? P.pid[rowAt<Real, FMT>(P, e + u + 1)] : -1;
             }
         }
+        if constexpr (FPL > 1) {
+            // the lane's adjacent fits in one access each, all kept as loaded
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t q = kk[u] * BP + GE::fit(sl, 0);
+                VecN<Real, FPL> v{};
+                if (val[u] && anyLane) v = *reinterpret_cast<const VecN<Real, FPL>*>(P.xb + q);
+                VecN<unsigned short, FPL> lo{};
+                if constexpr (sizeof(Real) == 4)
+                    if (val[u] && anyLane) lo = *reinterpret_cast<const VecN<unsigned short, FPL>*>(P.xbLo + q);
+#pragma unroll
+                for (int i = 0; i < FPL; ++i) {
+                    xv[u][i] = v.v[i];
+                    xl[u][i] = (Real)bf16Value(lo.v[i]);
+                }
+            }
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int i = 0; i < GE::FPL; ++i) {
-                const int f = sl + i * GE::SUB;
+                const int f = GE::fit(sl, i);
                 const bool go = val[u] && dl[i] != 0.0;
-                xv[u][i] = go ? P.xb[kk[u] * BP + f] : Real(0);
+                if constexpr (FPL == 1) {
+                    xv[u][i] = go ? P.xb[kk[u] * BP + f] : Real(0);
+                    if constexpr (sizeof(Real) == 4) xl[u][i] = go ? (Real)bf16Value(P.xbLo[kk[u] * BP + f]) : Real(0);
+                    else xl[u][i] = Real(0);
+                }
                 if constexpr (MODEL == M_COX) ww[u][i] = go ? weightOf(P, kk[u], f) : Real(0);
             }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (!val[u]) break;
+            VecN<Real, FPL> nvv;
+            VecN<unsigned short, FPL> nlv;
 #pragma unroll
             for (int i = 0; i < GE::FPL; ++i) {
+                if constexpr (FPL > 1) {
+                    nvv.v[i] = xv[u][i];
+                    if constexpr (sizeof(Real) == 4) nlv.v[i] = bf16Bits((float)xl[u][i]);
+                }
                 if (dl[i] == 0.0) continue;
-                const int f = sl + i * GE::SUB;
+                const int f = GE::fit(sl, i);
                 const Real old = xv[u][i];
-                const Real nv = old + (Real)dl[i] * xx[u];
-                P.xb[kk[u] * BP + f] = nv;
+                Real nv = old, nl = xl[u][i];
+                addXb<Real>(nv, nl, dl[i], xx[u]);
+                if constexpr (FPL > 1) {
+                    nvv.v[i] = nv;
+                    if constexpr (sizeof(Real) == 4) nlv.v[i] = bf16Bits((float)nl);
+                } else {
+                    P.xb[kk[u] * BP + f] = nv;
+                    if constexpr (sizeof(Real) == 4) P.xbLo[kk[u] * BP + f] = bf16Bits((float)nl);
+                }
                 if constexpr (GROUPED) {
                     Real diff = dexp(nv) - dexp(old);
                     if constexpr (MODEL == M_COX) diff *= ww[u][i];
                     acc[i] += (double)diff;
+                }
+            }
+            if constexpr (FPL > 1) {
+                if (anyLane) {
+                    const int64_t q = kk[u] * BP + GE::fit(sl, 0);
+                    *reinterpret_cast<VecN<Real, FPL>*>(P.xb + q) = nvv;
+                    if constexpr (sizeof(Real) == 4) *reinterpret_cast<VecN<unsigned short, FPL>*>(P.xbLo + q) = nlv;
                 }
             }
             if constexpr (GROUPED) {
@@ -513,7 +649,7 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
 #pragma unroll
                     for (int i = 0; i < GE::FPL; ++i) {
                         if (dl[i] == 0.0) continue;
-                        const int f = sl + i * GE::SUB;
+                        const int f = GE::fit(sl, i);
                         const size_t q = (size_t)p * BP + f;
                         const double dn = P.denom[q] + acc[i];
                         P.denom[q] = dn;
@@ -561,7 +697,7 @@ __device__ __forceinline__ void combinePlain(const Mon (&acc)[GE::FPL], Mon* sMo
             g += __shfl_xor(g, off, 64);
             h += __shfl_xor(h, off, 64);
         }
-        if (lane < GE::SUB) sMon[wave * BP + sl + i * GE::SUB] = Mon{g, h, 0, 0, 0, 0};
+        if (lane < GE::SUB) sMon[wave * BP + GE::fit(sl, i)] = Mon{g, h, 0, 0, 0, 0};
     }
     __syncthreads();
     for (int f = threadIdx.x; f < BP; f += BLOCK) {
@@ -1043,7 +1179,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     // ---- forward: D = segmented prefix of denom
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
-        const int f = sl + i * GE::SUB;
+        const int f = GE::fit(sl, i);
         Seg2 agg = z;
         fwdTiles(p0, p1, f, [&](int64_t, const Seg2& v) { agg = segCombine(agg, v); });
         s.scan[sg * BP + f] = agg;
@@ -1066,7 +1202,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     }
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
-        const int f = sl + i * GE::SUB;
+        const int f = GE::fit(sl, i);
         Seg2 run = segCombine(Seg2{s.carry[f], 0.0, 0, 0}, s.scan[sg * BP + f]);
         fwdTiles(p0, p1, f, [&](int64_t p, const Seg2& v) {
             run = segCombine(run, v);
@@ -1077,7 +1213,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     // ---- backward: R1, R2 = reverse segmented sums of nE/D, nE/D^2
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
-        const int f = sl + i * GE::SUB;
+        const int f = GE::fit(sl, i);
         Seg2 agg = z;
         bwdTiles(p0, p1, f, [&](int64_t, const Seg2& v) { agg = segCombine(agg, v); });
         s.scan[sg * BP + f] = agg;
@@ -1101,7 +1237,7 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
     }
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
-        const int f = sl + i * GE::SUB;
+        const int f = GE::fit(sl, i);
         Seg2 run = segCombine(Seg2{s.carry[f], s.carry2[f], 0, 0}, s.scan[sg * BP + f]);
         bwdTiles(p0, p1, f, [&](int64_t p, const Seg2& v) {
             run = segCombine(run, v);
@@ -1122,6 +1258,52 @@ namespace res {
 enum ObjOp : int { OBJ_LOGLIK = 0, OBJ_GRADIENT = 1, OBJ_LANGE = 2 };
 
 // This workgroup's share of the objective of every fit, into s.mon[f].g.
+template <int N>
+struct DVec {
+    double v[N];
+};
+
+// Row terms of the objective of this workgroup's rows for each fit of the lane, in
+// FP64 in every run (the convergence test compares objectives that differ by a
+// relative 1e-6 or less; FP32 runs add the low part of xb). It runs once per cycle
+// and is kept out of line, so the registers of its FP64 exp do not add to those the
+// coordinate loop holds.
+template <typename Real, int MODEL, int BP>
+__device__ __attribute__((noinline)) DVec<GeoM<BP, MODEL>::FPL> objectiveRows(const Params<Real>& P, int op) {
+    using GE = GeoM<BP, MODEL>;
+    const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
+    const int w = P.w;
+    DVec<GE::FPL> acc;
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) acc.v[i] = 0.0;
+    const int64_t a = P.rowStart[w], b = P.rowStart[w + 1];
+    const int64_t c = (b - a + GE::NSG - 1) / GE::NSG;
+    int64_t n0 = a + sg * c, n1 = n0 + c;
+    if (n0 > b) n0 = b;
+    if (n1 > b) n1 = b;
+    for (int64_t k = n0; k < n1; ++k) {
+        const Real yk = P.y[k];
+        const int yi = (int)yk;
+#pragma unroll
+        for (int i = 0; i < GE::FPL; ++i) {
+            const int f = GE::fit(sl, i);
+            if (f >= P.B) continue;
+            double x = (double)P.xb[k * BP + f];
+            if constexpr (sizeof(Real) == 4) x += (double)bf16Value(P.xbLo[k * BP + f]);
+            const double wt = (double)weightOf(P, k, f);
+            if (op == OBJ_GRADIENT) {
+                acc.v[i] += x * (double)yk * wt;
+                continue;
+            }
+            double cterm = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
+            if constexpr (MODEL == M_LOGISTIC) cterm -= log(1.0 + exp(x));
+            if constexpr (MODEL == M_POISSON) cterm -= exp(x);
+            acc.v[i] += cterm * wt;
+        }
+    }
+    return acc;
+}
+
 template <typename Real, int MODEL, int BP>
 __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, Shared<BP>& s) {
     using GE = GeoM<BP, MODEL>;
@@ -1129,33 +1311,10 @@ __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, 
     const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
     const int w = P.w;
     double acc[GE::FPL];
+    {   // rows (once per cycle, in a function of its own: see objectiveRows)
+        const DVec<GE::FPL> r = objectiveRows<Real, MODEL, BP>(P, op);
 #pragma unroll
-    for (int i = 0; i < GE::FPL; ++i) acc[i] = 0.0;
-    {   // rows
-        const int64_t a = P.rowStart[w], b = P.rowStart[w + 1];
-        const int64_t c = (b - a + GE::NSG - 1) / GE::NSG;
-        int64_t n0 = a + sg * c, n1 = n0 + c;
-        if (n0 > b) n0 = b;
-        if (n1 > b) n1 = b;
-        for (int64_t k = n0; k < n1; ++k) {
-            const Real yk = P.y[k];
-            const int yi = (int)yk;
-#pragma unroll
-            for (int i = 0; i < GE::FPL; ++i) {
-                const int f = sl + i * GE::SUB;
-                if (f >= P.B) continue;
-                const double x = (double)P.xb[k * BP + f];
-                const double wt = (double)weightOf(P, k, f);
-                if (op == OBJ_GRADIENT) {
-                    acc[i] += x * (double)yk * wt;
-                    continue;
-                }
-                double cterm = (MODEL == M_COX) ? (yi != 1 ? 0.0 : x) : yi * x;
-                if constexpr (MODEL == M_LOGISTIC) cterm -= log(1.0 + (double)dexp((Real)x));
-                if constexpr (MODEL == M_POISSON) cterm -= (double)dexp((Real)x);
-                acc[i] += cterm * wt;
-            }
-        }
+        for (int i = 0; i < GE::FPL; ++i) acc[i] = r.v[i];
     }
     if constexpr (GROUPED) {
         if (op != OBJ_GRADIENT) {
@@ -1168,7 +1327,7 @@ __device__ __forceinline__ void objectivePartial(const Params<Real>& P, int op, 
             for (int64_t p = n0; p < n1; ++p) {
 #pragma unroll
                 for (int i = 0; i < GE::FPL; ++i) {
-                    const int f = sl + i * GE::SUB;
+                    const int f = GE::fit(sl, i);
                     if (f >= P.B) continue;
                     const double nw = P.nW[p * BP + f];
                     if (nw != 0.0) acc[i] -= nw * log(den[p * BP + f]);
@@ -1196,7 +1355,7 @@ __device__ __forceinline__ void initPids(const Params<Real>& P) {
     for (int64_t p = n0; p < n1; ++p) {
 #pragma unroll
         for (int i = 0; i < GE::FPL; ++i) {
-            const int f = sl + i * GE::SUB;
+            const int f = GE::fit(sl, i);
             double dn = 0.0, nw = 0.0;
             if (f < P.B) {
                 for (int k = P.pstart[p]; k < P.pstart[p + 1]; ++k) {
@@ -1335,6 +1494,9 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     // geometry; wider teams would hold too many registers)
     constexpr bool FAST = BP <= 4 || (!GROUPED && BP <= 16);
     constexpr int UF = 4;   // entries per sub-group kept in registers
+    // compensated xb: the low parts travel with the registers for independent-row
+    // models; grouped models, whose update also walks per-pid state, fetch them there
+    constexpr bool LOW = sizeof(Real) == 4 && !GROUPED;
     auto fitsBatch = [&](const Meta& m) {   // every sub-group chunk of m has <= UF entries
         return (m.b - m.a + GE::NSG - 1) / GE::NSG <= UF;
     };
@@ -1427,7 +1589,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 bool on[GE::FPL];
 #pragma unroll
                 for (int i = 0; i < GE::FPL; ++i) {
-                    const int f = sl + i * GE::SUB;
+                    const int f = GE::fit(sl, i);
                     on[i] = f < P.B && s.active[f];
                 }
                 // a chunk of at most U entries stays in registers until the update
@@ -1441,13 +1603,13 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 if (FAST && fast) {
                     if (!GROUPED && curRows) {
                         bt = nb;   // row ids and values fetched during the previous coordinate
-                        loadState<Real, MODEL, BP, UF>(P, on, bt);
+                        loadState<Real, MODEL, BP, UF, LOW>(P, on, bt);
                     } else {
                         switch (fm) {
-                        case F_INDICATOR: loadBatch<Real, MODEL, BP, F_INDICATOR, UF>(P, j, n0, n1, on, bt); break;
-                        case F_SPARSE: loadBatch<Real, MODEL, BP, F_SPARSE, UF>(P, j, n0, n1, on, bt); break;
-                        case F_DENSE: loadBatch<Real, MODEL, BP, F_DENSE, UF>(P, j, n0, n1, on, bt); break;
-                        default: loadBatch<Real, MODEL, BP, F_INTERCEPT, UF>(P, j, n0, n1, on, bt); break;
+                        case F_INDICATOR: loadBatch<Real, MODEL, BP, F_INDICATOR, UF, LOW>(P, j, n0, n1, on, bt); break;
+                        case F_SPARSE: loadBatch<Real, MODEL, BP, F_SPARSE, UF, LOW>(P, j, n0, n1, on, bt); break;
+                        case F_DENSE: loadBatch<Real, MODEL, BP, F_DENSE, UF, LOW>(P, j, n0, n1, on, bt); break;
+                        default: loadBatch<Real, MODEL, BP, F_INTERCEPT, UF, LOW>(P, j, n0, n1, on, bt); break;
                         }
                     }
                     int prevSeg = 0;
@@ -1470,7 +1632,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 prefetchRows();
                 if constexpr (GROUPED) {
 #pragma unroll
-                    for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + sl + i * GE::SUB] = acc[i];
+                    for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + GE::fit(sl, i)] = acc[i];
                     __syncthreads();
                     combineSubgroups<BP>(s.mon);
                 } else {
@@ -1484,7 +1646,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 int any = 0;
 #pragma unroll
                 for (int i = 0; i < GE::FPL; ++i) {
-                    const int f = sl + i * GE::SUB;
+                    const int f = GE::fit(sl, i);
                     dl[i] = f < P.B ? s.out[4 * f] : 0.0;
                     any |= (dl[i] != 0.0);
                 }
@@ -1493,7 +1655,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     for (int f = 0; f < P.B; ++f) nUpd += (s.out[4 * f] != 0.0) ? 1 : 0;
                 if (__syncthreads_or(any)) {
                     if (FAST && fast) {
-                        applyBatch<Real, MODEL, BP, UF>(P, bt, dl);
+                        applyBatch<Real, MODEL, BP, UF, LOW>(P, bt, dl);
                     } else {
                         switch (fm) {
                         case F_INDICATOR: updatePass<Real, MODEL, BP, F_INDICATOR>(P, j, n0, n1, dl); break;
@@ -1522,9 +1684,13 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
                                              ((fm == F_SPARSE || fm == F_DENSE) ? sizeof(Real) : 0.0) +
                                              ((P.rowFold || P.kw) ? 4.0 : 0.0));
-                    double by = base + n * nAct * sizeof(Real) + (GROUPED ? n * (8.0 + 16.0 * nAct) : 0.0);
+                    // (FP32 runs also move the low part of xb: read with the gathers on the
+                    // fast path, read and written by every update)
+                    constexpr double lowB = sizeof(Real) == 4 ? 2.0 : 0.0;
+                    double by = base + n * nAct * (sizeof(Real) + (fast ? lowB : 0.0)) +
+                                (GROUPED ? n * (8.0 + 16.0 * nAct) : 0.0);
                     if (nUpd > 0) {
-                        by += base + n * nUpd * 2.0 * sizeof(Real) + (GROUPED ? n * (4.0 + 16.0 * nUpd) : 0.0);
+                        by += base + n * nUpd * 2.0 * (sizeof(Real) + lowB) + (GROUPED ? n * (4.0 + 16.0 * nUpd) : 0.0);
                         if (MODEL == M_COX) by += (double)P.N * P.B * 56.0;
                     }
                     loc[0] += by;
