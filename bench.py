@@ -30,12 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-# HBM bytes moved by kFindMode per algorithmic byte on the default workload (64 fits,
-# scale 1): two rocprofv3 --pmc passes over this script's own timed step, FETCH_SIZE x 2
-# plus WRITE_SIZE x 1 (the factors a 1 GiB device copy reads back), summed over the
-# step's launches and divided by the bytes the kernel counts.  The measurement and its
-# commands are in profiles/kFindMode_poisson_K1e6_J2e4_batch64_traffic.md.
-TRAFFIC_PER_ALGORITHMIC_BYTE = 1.2422
+# HBM bytes per one-cycle launch of the default batch (64 fits, teams of 16 fits x 64
+# workgroups, full size): two rocprofv3 --pmc passes over `--workload traffic`,
+# FETCH_SIZE x 2 plus WRITE_SIZE x 1 (the factors the copy kernel over 1 GiB reads
+# back); the commands and per-dispatch values are in
+# profiles/kFindMode_poisson_K1e6_J2e4_batch64_traffic.md.
+TRAFFIC_BYTES_PER_CYCLE_B64 = 6.169e11
 
 
 def log(msg):
@@ -207,6 +207,10 @@ def run_poisson(args):
     algo_bytes = float(np.mean([poisson_cycle_bytes(d, s_bytes, a) for a in active_per_step]))
     achieved = algo_bytes / (launch_ms / 1e3) / 1e9
     kernel_bytes = ks["algorithmic_bytes"] / max(ks["launches"], 1)
+    # counter-measured traffic applies to the shape it was measured on
+    measured_shape = (B == 64 and args.scale == 1.0 and lay["fits_per_team"] == 16 and
+                      lay["workgroups_per_team"] == 64 and min(active_per_step) == B)
+    traffic = TRAFFIC_BYTES_PER_CYCLE_B64 if measured_shape else None
 
     # one member of the mix: variance closest to 1.0, fold 0
     ref_i = int(np.argmin([abs(math.log(v)) + 10 * h for v, h in zip(var_list, held)]))
@@ -348,7 +352,7 @@ def run_poisson(args):
         "speedup_vs_plain": (value / plain["fits_per_sec"]) if plain else None,
         "fp32_vs_fp64": fp64,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "ccd::res::kFindMode<float, POISSON, BP>", "launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "bytes_rule": "per indicator entry and cycle 8 + 7 s B (gradient 4 + s B, update 4 + 6 s B), "
@@ -468,41 +472,38 @@ def run_serving(args):
 
 
 def run_traffic(args):
-    """Short run for HBM-traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
-    a 1 GiB device copy (known traffic, for scaling the counters), then one launch of
-    2 cycles of the headline batch (64 Poisson fits, teams of 16 fits x 64 workgroups)
-    whose algorithmic bytes are printed for comparison."""
-    import torch
+    """Short run for the HBM-traffic counters (rocprofv3 --pmc FETCH_SIZE, then
+    WRITE_SIZE): the copy kernel of ccd_copy_bandwidth over 1 GiB buffers (two
+    launches of known traffic, for scaling the counters), then three single-cycle
+    steps of the headline batch (64 Poisson fits, teams of 16 fits x 64 workgroups),
+    whose algorithmic bytes per cycle are printed for comparison."""
     from cyclops_amd import _abi
     from cyclops_amd import data as D
     from cyclops_amd import engine as E
     from cyclops_amd.control import create_control
 
-    n = 1 << 28   # 2^28 float32 = 1 GiB
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    log(f"copy kernel: {n * 4} bytes read and {n * 4} bytes written")
-    del src, dst
-    torch.cuda.empty_cache()
+    nbytes = 1 << 30
+    E.copy_bandwidth(0, nbytes, 1)
+    log(f"copy kernel: 2 launches, each reading {nbytes} bytes and writing {nbytes} bytes")
     os.environ["CCD_BATCH_LAYOUT"] = "16,64"
-    os.environ["CCD_CYCLES_PER_LAUNCH"] = "2"
     d, _ = D.make_config(1, scale=args.scale)
     rf = E.assign_folds(d.K, None, 4, _abi.SELECT_BY_ROW, 1001)
     grid = np.exp(np.linspace(math.log(0.1), math.log(10.0), 16))
     var_list = [float(v) for v in grid for _ in range(4)]
     held = [f for _ in range(16) for f in range(4)]
     eng = E.Engine(d, _abi.FP32)
+    run = eng.begin_batch(var_list, "normal", create_control(convergence_type="mittal", tolerance=1e-6),
+                          held_out_fold=held, row_fold=rf)
     eng.reset_kernel_stats()
-    eng.fit_batch(var_list, "normal", create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=2),
-                  held_out_fold=held, row_fold=rf)
+    for _ in range(3):
+        run.step(1)
     ks = eng.kernel_stats()
+    run.abandon()
     eng.close()
-    print(json.dumps({"copy_bytes_each_way": n * 4, "kfindmode_launches": ks["launches"],
-                      "kfindmode_algorithmic_bytes": ks["algorithmic_bytes"], "kfindmode_device_ms": ks["device_ms"]}),
-          flush=True)
+    print(json.dumps({"copy_bytes_each_way": nbytes, "kfindmode_launches": ks["launches"],
+                      "kfindmode_kernel_counted_bytes": ks["algorithmic_bytes"],
+                      "algorithmic_bytes_per_cycle": poisson_cycle_bytes(d, 4.0, len(var_list)),
+                      "kfindmode_device_ms": ks["device_ms"]}), flush=True)
 
 
 def run_other(args):
