@@ -39,8 +39,11 @@
  * logistic fit, brute-force Breslow partial likelihood and its derivatives on the
  * reference's inline Cox tables, finite differences of the log-likelihood for every
  * model and column format, Laplace KKT conditions, the SAS -2 log-likelihood stored
- * with the reference's 17-variable logistic fixture, and CLR against an
- * independent Newton solver on the reference's infert fixture.
+ * with the reference's 17-variable logistic fixture, CLR against an independent
+ * Newton solver on the reference's infert fixture, and the held-out predictive
+ * log-likelihood, gradient objective and predictive estimates against numpy
+ * restatements from a dense X (Cox held-out risk sets by O(K^2) brute force), their
+ * unit-weight identities with the log-likelihood and their additivity over folds.
  */
 #include "../include/cyclops_amd.h"
 
