@@ -144,9 +144,11 @@ static std::shared_ptr<DevData<Real>> makeDevData(const ccd_data_desc& d, int de
     dd->hasOffset = d.has_offset != 0;
     if (d.K <= 0 || d.J <= 0) throw std::invalid_argument("K and J must be positive");
     if (d.K >= (int64_t)INT32_MAX) throw std::invalid_argument("K must fit int32 row ids");
+    // other model types: the factory's nullptr (AbstractModelSpecifics.cpp:128-154), which
+    // RcppCyclopsInterface.cpp:890-892 reports as "Invalid model type." -> CCD_ERR_UNSUPPORTED
     if (!(d.model_type == M_POISSON || d.model_type == M_LOGISTIC || d.model_type == M_CLR ||
           d.model_type == M_COX))
-        throw std::invalid_argument("Invalid model type.");
+        throw std::logic_error("Invalid model type.");
     if (!d.format || !d.col_ptr || !d.y) throw std::invalid_argument("format, col_ptr and y are required");
     const int64_t K = d.K;
     const int J = d.J;

@@ -231,3 +231,23 @@ def test_reference_fixtures_on_device(E, name, mode):
         assert rg["log_likelihood"] * 2 == pytest.approx(-1578.046, rel=1e-4)
     if name == "infert":  # clogit estimates used by test-smallCLR.R
         assert bg == pytest.approx([1.98587, 1.40901], rel=1e-4)
+
+
+def test_create_reports_unsupported_models_like_the_factory(E):
+    """A model type the engine does not implement (here NORMAL = 0, least squares) is
+    the factory's nullptr, reported as "Invalid model type." with CCD_ERR_UNSUPPORTED;
+    a precision outside PrecisionType's FP64 / FP32 is an invalid argument."""
+    import ctypes as C
+    import _cases
+    d = _cases.logistic()
+    L = E.lib()
+    h = C.c_void_p()
+    desc = d.desc(_abi.FP64)
+    desc.model_type = 0
+    assert L.ccd_create(C.byref(desc), 0, C.byref(h)) == _abi.ERR_UNSUPPORTED
+    assert L.ccd_last_error().decode() == "Invalid model type."
+    desc = d.desc(2)   # PrecisionType::FP16
+    assert L.ccd_create(C.byref(desc), 0, C.byref(h)) == _abi.ERR_INVALID
+    desc = d.desc(_abi.FP32_BITS)   # the bit count is accepted
+    assert L.ccd_create(C.byref(desc), 0, C.byref(h)) == 0
+    L.ccd_destroy(h)
