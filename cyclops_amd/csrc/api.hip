@@ -471,6 +471,13 @@ int ccd_grad_hess(ccd_handle h, int j, double* g, double* hh, int use) {
         E(h).gradHess(j, g, hh, use != 0);
     });
 }
+int ccd_grad_hess_noalloc(ccd_handle h, int j, double* g, double* hh, int use) {
+    return guard([&] {
+        if (!g || !hh) throw std::invalid_argument("null argument");
+        if (j < 0 || j >= E(h).J()) throw std::invalid_argument("column out of range");
+        E(h).gradHessNoAlloc(j, g, hh, use != 0);
+    });
+}
 int ccd_update_xbeta(ccd_handle h, int j, double delta, int use) {
     return guard([&] {
         if (j < 0 || j >= E(h).J()) throw std::invalid_argument("column out of range");

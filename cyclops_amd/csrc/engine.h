@@ -94,6 +94,8 @@ public:
     virtual void remainingStats(bool useW) = 0;
     virtual void numerator(int j, bool useW) = 0;
     virtual void gradHess(int j, double* g, double* h, bool useW) = 0;
+    // gradHess with its reduction buffers in the static scratch of the device
+    virtual void gradHessNoAlloc(int j, double* g, double* h, bool useW) = 0;
     virtual void updateXBeta(double delta, int j, bool useW) = 0;
     virtual double logLik(bool useW) = 0;
     virtual double predLogLik(const double* w) = 0;
@@ -126,6 +128,10 @@ public:
 };
 
 EngineBase* createEngine(const ccd_data_desc& d, int device);
+
+// Device address of the no-allocation scratch of `device`, which must be current;
+// looked up once per device (engine.hip).
+void* noAllocScratch(int device);
 
 // PrecisionType of a data description (FP64 = 0, FP32 = 1, or the bit counts 64 / 32)
 // as CCD_FP64 / CCD_FP32; anything else is refused.

@@ -133,6 +133,36 @@ void DevData<Real>::buildRowMajor(hipStream_t s) {
     bytes += b;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (ccd_grad_hess_noalloc): one copy per
+// device, held until the process ends. Layout: completion counter at 0, result
+// (g, h) at 256, per-block partials from 512. In host code `scratch` names a host
+// stand-in, not device memory: only the address hipGetSymbolAddress returns for the
+// current device is ever passed to a kernel or a copy.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[CCD_NOALLOC_SCRATCH_BYTES];
+
+__global__ void kZeroCounter(unsigned* c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+void* noAllocScratch(int device) {
+    static std::mutex mu;
+    static std::vector<void*> ptrs;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptrs.size() <= device) ptrs.resize(device + 1, nullptr);
+    if (!ptrs[device]) {   // once per device, with the device current (the caller set it)
+        int cur = -1;
+        CCD_HIP(hipGetDevice(&cur));
+        if (cur != device) throw std::logic_error("noAllocScratch: device is not current");
+        void* p = nullptr;
+        CCD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array");
+        ptrs[device] = p;
+    }
+    return ptrs[device];
+}
+
 // Validate the description and copy it to HBM.
 template <typename Real>
 static std::shared_ptr<DevData<Real>> makeDevData(const ccd_data_desc& d, int device) {
@@ -541,7 +571,9 @@ public:
     }
 
     // ---- computeGradientAndHessian (ModelSpecifics.hpp:710-766, 891-1201) ----
-    void gradHessAsync(int j, bool useW) {
+    // Launch of the column pass of j; per-block partials, the completion counter (zero on
+    // entry, left at zero) and the result (g, h) go to the given buffers.
+    void gradHessAsync(int j, bool useW, Partial* parts, unsigned* counter, double* dOut) {
         const bool W = useW && weighted;
         const Col<Real> c = col(j);
         const bool indep = D->model == M_LOGISTIC || D->model == M_POISSON;
@@ -574,7 +606,7 @@ public:
         if (nonZeros(j) == 0) { *g = 0.0; *h = 0.0; return; }
         CCD_HIP(hipSetDevice(D->device));
         if (timing) CCD_HIP(hipEventRecord(ev0, s));
-        gradHessAsync(j, useW);
+        gradHessAsync(j, useW, parts, counter, dOut);
         if (timing) CCD_HIP(hipEventRecord(ev1, s));
         CCD_HIP(hipMemcpyAsync(hOut, dOut, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         CCD_HIP(hipStreamSynchronize(s));
@@ -585,6 +617,30 @@ public:
             counters.launches += 1;
             counters.bytes += gradBytes(j);
         }
+        *g = hOut[0];
+        *h = hOut[1];
+    }
+
+    // gradHess with the reduction buffers in the library's static scratch of this device
+    // (ccd_grad_hess_noalloc): allocates nothing.
+    void gradHessNoAlloc(int j, double* g, double* h, bool useW) override {
+        if (nonZeros(j) == 0) { *g = 0.0; *h = 0.0; return; }
+        CCD_HIP(hipSetDevice(D->device));
+        char* base = static_cast<char*>(noAllocScratch(D->device));
+        unsigned* cnt = reinterpret_cast<unsigned*>(base);
+        double* out = reinterpret_cast<double*>(base + 256);
+        Partial* pr = reinterpret_cast<Partial*>(base + 512);
+        const bool indep = D->model == M_LOGISTIC || D->model == M_POISSON;
+        const size_t need = 512 + sizeof(Partial) * (size_t)(indep ? MAX_GRAD_BLOCKS : MAX_RED_BLOCKS);
+        if (need > (size_t)CCD_NOALLOC_SCRATCH_BYTES)
+            throw std::invalid_argument("the column pass needs " + std::to_string(need) +
+                                        " bytes of scratch; the no-allocation entry point has " +
+                                        std::to_string(CCD_NOALLOC_SCRATCH_BYTES));
+        kZeroCounter<<<1, 64, 0, s>>>(cnt);   // the last call's contents stay in the array
+        CCD_HIP(hipGetLastError());
+        gradHessAsync(j, useW, pr, cnt, out);
+        CCD_HIP(hipMemcpyAsync(hOut, out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        CCD_HIP(hipStreamSynchronize(s));
         *g = hOut[0];
         *h = hOut[1];
     }

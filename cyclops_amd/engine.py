@@ -53,6 +53,7 @@ def lib():
         L.ccd_remaining_stats.argtypes = [vp, C.c_int]
         L.ccd_numerator.argtypes = [vp, C.c_int, C.c_int]
         L.ccd_grad_hess.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+        L.ccd_grad_hess_noalloc.argtypes = [vp, C.c_int, dp, dp, C.c_int]
         L.ccd_update_xbeta.argtypes = [vp, C.c_int, C.c_double, C.c_int]
         L.ccd_loglik.argtypes = [vp, C.c_int, dp]
         L.ccd_pred_loglik.argtypes = [vp, dp, dp]

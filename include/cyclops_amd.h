@@ -166,6 +166,19 @@ int ccd_remaining_stats(ccd_handle h, int use_weights);
 int ccd_numerator(ccd_handle h, int j, int use_weights);
 /* computeGradientAndHessian(int, double*, double*, bool)  AbstractModelSpecifics.h:67 */
 int ccd_grad_hess(ccd_handle h, int j, double* gradient, double* hessian, int use_weights);
+/* ccd_grad_hess without allocation: the same gradient and Hessian, from the same
+ * kernels and launch shape, with the per-block partial sums, the completion counter
+ * and the result in a static device array of the library (CCD_NOALLOC_SCRATCH_BYTES
+ * per device, held until the process ends) instead of the engine's buffers, copied
+ * back through the engine's pinned result buffer. The call allocates no device or
+ * host memory. No counterpart in the reference: for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * allocation). Every call on a device shares that device's array: calls that
+ * overlap on one device race, so run them on one host thread or serialise them. A
+ * column pass that would need more scratch than the array holds is refused with
+ * CCD_ERR_INVALID. */
+#define CCD_NOALLOC_SCRATCH_BYTES 65536
+int ccd_grad_hess_noalloc(ccd_handle h, int j, double* gradient, double* hessian, int use_weights);
 /* updateXBeta(double, int, bool)  AbstractModelSpecifics.h:77 */
 int ccd_update_xbeta(ccd_handle h, int j, double delta, int use_weights);
 /* getLogLikelihood(bool)  AbstractModelSpecifics.h:87 */
