@@ -335,9 +335,11 @@ static void crossValidate(EngineBase& eng, const ccd_fit_args& a, const ccd_cv_a
     ccd_batch_args b{nFits, var.data(), held.data(), rowFold.data(), nullptr};
     ccd_fit_args common = a;
     common.weights = nullptr;
-    eng.fitBatch(common, b, betas.data(), res.data());
-    // predictive log-likelihood of each fit on its held-out rows
-    std::vector<double> w(K), comp(K);
+    // every fit's predictive log-likelihood on its held-out rows is formed on the device
+    // right after its run, from the xb the run leaves in HBM
+    // (CyclicCoordinateDescent::getNewPredictiveLogLikelihood, :319-355)
+    std::vector<double> pred(nFits, 0.0);
+    eng.fitBatch(common, b, betas.data(), res.data(), true, pred.data());
     double bestValue = -std::numeric_limits<double>::infinity(), bestPoint = points[0];
     for (int step = 0; step < cv.grid_size; ++step) {
         double total = 0.0;
@@ -345,16 +347,7 @@ static void crossValidate(EngineBase& eng, const ccd_fit_args& a, const ccd_cv_a
         for (int f = 0; f < cv.fold; ++f) {
             const int i = step * cv.fold + f;
             double v = std::numeric_limits<double>::quiet_NaN();
-            if (res[i].return_flag == CCD_SUCCESS) {
-                for (int64_t k = 0; k < K; ++k) {
-                    w[k] = rowFold[k] == f ? 0.0 : 1.0;
-                    comp[k] = 1.0 - w[k];
-                }
-                eng.setWeights(w.data(), true);
-                eng.computeXBeta(betas.data() + (size_t)i * J);
-                eng.remainingStats(true);
-                v = eng.predLogLik(comp.data());
-            }
+            if (res[i].return_flag == CCD_SUCCESS) v = pred[i];
             if (predLL) predLL[i] = v;
             if (v == v) { total += v; ++count; }
         }

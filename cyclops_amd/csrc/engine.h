@@ -112,8 +112,10 @@ public:
     virtual void fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) = 0;
     // autotune: time candidate team shapes on the first batch of each size (callers
     // issuing many small batches of varying size, such as profiling, pass false)
+    // predHeldOut [n_fits] (optional, needs row_fold): each fit's predictive
+    // log-likelihood on its held-out rows, formed on the device from the run's xb
     virtual void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas,
-                          ccd_fit_result* r, bool autotune = true) = 0;
+                          ccd_fit_result* r, bool autotune = true, double* predHeldOut = nullptr) = 0;
     // The same fits as fitBatch, stepped by the caller; the engine runs no other
     // resident fit until the returned batch has finished or is deleted.
     virtual SteppedBatch* beginStepped(const ccd_fit_args& a, const ccd_batch_args& b, const double* betas0) = 0;

@@ -777,7 +777,7 @@ public:
     // resident.hip
     void fitResident(const ccd_fit_args& a, double* beta, ccd_fit_result& r) override;
     void fitBatch(const ccd_fit_args& a, const ccd_batch_args& b, double* betas, ccd_fit_result* r,
-                  bool autotune) override;
+                  bool autotune, double* predHeldOut) override;
     SteppedBatch* beginStepped(const ccd_fit_args& a, const ccd_batch_args& b, const double* betas0) override;
     bool steppedOpen() const override { return stepped != nullptr; }
     // fisher.inl

@@ -172,6 +172,9 @@ struct ResWork {
     // per batch
     Real* xb = nullptr;
     unsigned short* xbLo = nullptr;   // FP32 runs: low part of xb (bfloat16 bits)
+    double* predBuf = nullptr;        // held-out predictive pass (grow-only)
+    size_t predCap = 0;
+    double* timeDev = nullptr;        // Cox survival times (held-out risk sets)
     double *denom = nullptr, *Dacc = nullptr, *R1 = nullptr, *R2 = nullptr, *nW = nullptr;
     double *beta = nullptr, *hDelta = nullptr, *var = nullptr, *initBound = nullptr;
     unsigned char *exclude = nullptr, *fixed = nullptr;
@@ -195,7 +198,7 @@ struct ResWork {
     std::map<int, std::array<int, 3>> tuned;   // batch size -> {Bt, T, nTeams} picked by timing
 
     void freeBatch() {
-        dfree(xb); dfree(xbLo); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
+        dfree(xb); dfree(xbLo); dfree(predBuf); predCap = 0; dfree(timeDev); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
@@ -825,17 +828,249 @@ static void gatherRun(ResRun<Real>& R, double* betas, int* flags, int* iters, do
                      st8[4] / R.rateKHz);
 }
 
+// ---------------------------------------------------------------------------
+// Held-out predictive log-likelihood of every fit of a finished run, from the xb
+// the run leaves in HBM (getPredictiveLogLikelihood, ModelSpecifics.hpp:652-690,
+// with the held-out rows of fit f, rowFold[k] == held[f], as the weights; the terms
+// of ModelSpecifics.h: CLR :867-873, LR :1002-1013, Breslow Cox :1243-1256,
+// Poisson :1583-1594). One pass for every fit of a team replaces, per fit, the
+// reference's setWeights + computeXBeta + computeRemainingStatistics round trip.
+// Everything is summed in FP64 from xb (plus its low part in FP32 runs), in a
+// fixed order.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__device__ __forceinline__ double xbOf(const Real* xb, const unsigned short* lo, size_t q) {
+    double x = (double)xb[q];
+    if (lo) x += (double)bf16Value(lo[q]);
+    return x;
+}
+
+// LR / PR: one group per row; CLR: one group per stratum (rows [gs[g], gs[g+1])).
+// Thread (group, lane); block partials per lane: part[block * BP + lane].
+template <typename Real, int MODEL>
+__global__ __launch_bounds__(BLOCK) void kPredGroups(int64_t nGroups, const int* __restrict__ gs, int BP, int Bt,
+                                                    const Real* __restrict__ xb, const unsigned short* __restrict__ lo,
+                                                    const Real* __restrict__ y, const int* __restrict__ rowFold,
+                                                    const int* __restrict__ held, double* __restrict__ part) {
+    __shared__ double sh[BLOCK];
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t g = t / BP;
+    const int f = (int)(t % BP);
+    double c = 0.0;
+    if (g < nGroups && f < Bt) {
+        const int hf = held[f];
+        if constexpr (MODEL == M_CLR) {
+            const int64_t a = gs[g], b = gs[g + 1];
+            double den = 0.0;
+            for (int64_t k = a; k < b; ++k) den += exp(xbOf(xb, lo, (size_t)k * BP + f));
+            const double lden = log(den);
+            for (int64_t k = a; k < b; ++k)
+                if (rowFold[k] == hf && y[k] != Real(0)) c += (double)y[k] * (xbOf(xb, lo, (size_t)k * BP + f) - lden);
+        } else {
+            const int64_t k = g;
+            if (rowFold[k] == hf) {
+                const double x = xbOf(xb, lo, (size_t)k * BP + f);
+                if constexpr (MODEL == M_LOGISTIC) c = (double)y[k] * (x - log1p(exp(x)));
+                else c = (double)y[k] * x - exp(x);
+            }
+        }
+    }
+    sh[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x < BP) {   // groups of the block in order
+        double sum = 0.0;
+        for (int i = threadIdx.x; i < BLOCK; i += BP) sum += sh[i];
+        part[(size_t)blockIdx.x * BP + threadIdx.x] = sum;
+    }
+}
+
+// Per lane, the block partials summed in block order: out[lane].
+__global__ void kPredSum(int64_t nParts, int BP, int Bt, const double* __restrict__ part, double* __restrict__ out) {
+    const int f = threadIdx.x;
+    if (f >= Bt) return;
+    double s = 0.0;
+    for (int64_t b = 0; b < nParts; ++b) s += part[(size_t)b * BP + f];
+    out[f] = s;
+}
+
+// Breslow Cox over held-out risk sets: rows sorted (stratum, time desc, y asc), so the
+// risk set of an event run (tied events of one time, the last rows of that time) is
+// every row from the stratum's start to the run's end; D = sum of w exp(xb) over it,
+// w = held-out indicator. Chunks [cs[c], cs[c+1]) never split an event run.
+// Pass 1: per (chunk, lane) the sum after the chunk's last stratum start.
+template <typename Real>
+__global__ __launch_bounds__(BLOCK) void kPredCoxTail(int nChunks, const int* __restrict__ cs, int BP, int Bt,
+                                                     const int* __restrict__ strat, const Real* __restrict__ xb,
+                                                     const unsigned short* __restrict__ lo,
+                                                     const int* __restrict__ rowFold, const int* __restrict__ held,
+                                                     double* __restrict__ tail) {
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int c = (int)(t / BP), f = (int)(t % BP);
+    if (c >= nChunks || f >= Bt) return;
+    const int hf = held[f];
+    double s = 0.0;
+    for (int k = cs[c]; k < cs[c + 1]; ++k) {
+        if (k > cs[c] && strat[k] != strat[k - 1]) s = 0.0;
+        if (rowFold[k] == hf) s += exp(xbOf(xb, lo, (size_t)k * BP + f));
+    }
+    tail[(size_t)c * BP + f] = s;
+}
+
+// Pass 2: the carry into each chunk (the sum since its stratum's start), per lane.
+// startsNew[c]: chunk c begins a stratum; inner[c]: a stratum begins inside chunk c.
+__global__ void kPredCoxCarry(int nChunks, int BP, int Bt, const unsigned char* __restrict__ startsNew,
+                              const unsigned char* __restrict__ inner, const double* __restrict__ tail,
+                              double* __restrict__ carry) {
+    const int f = threadIdx.x;
+    if (f >= Bt) return;
+    double cin = 0.0;
+    for (int c = 0; c < nChunks; ++c) {
+        if (startsNew[c]) cin = 0.0;
+        carry[(size_t)c * BP + f] = cin;
+        cin = inner[c] ? tail[(size_t)c * BP + f] : cin + tail[(size_t)c * BP + f];
+    }
+}
+
+// Pass 3: walk the chunk with its carry; each event run adds sum(xb) - n log D.
+template <typename Real>
+__global__ __launch_bounds__(BLOCK) void kPredCoxWalk(int nChunks, const int* __restrict__ cs, int BP, int Bt,
+                                                     const int* __restrict__ strat, const double* __restrict__ time,
+                                                     const Real* __restrict__ y, const Real* __restrict__ xb,
+                                                     const unsigned short* __restrict__ lo,
+                                                     const int* __restrict__ rowFold, const int* __restrict__ held,
+                                                     const double* __restrict__ carry, double* __restrict__ part) {
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int c = (int)(t / BP), f = (int)(t % BP);
+    if (c >= nChunks || f >= Bt) return;
+    const int hf = held[f];
+    double run = carry[(size_t)c * BP + f], sumX = 0.0, nEv = 0.0, out = 0.0;
+    const int a = cs[c], b = cs[c + 1];
+    for (int k = a; k < b; ++k) {
+        if (k > a && strat[k] != strat[k - 1]) run = 0.0;
+        const bool w = rowFold[k] == hf;
+        const double x = xbOf(xb, lo, (size_t)k * BP + f);
+        if (w) run += exp(x);
+        if (y[k] != Real(0) && w) { sumX += x; nEv += 1.0; }
+        const bool more = k + 1 < b && y[k] != Real(0) && y[k + 1] != Real(0) && time[k + 1] == time[k] &&
+                          strat[k + 1] == strat[k];
+        if (!more && y[k] != Real(0)) {   // end of an event run
+            if (nEv > 0.0) out += sumX - nEv * log(run);
+            sumX = 0.0;
+            nEv = 0.0;
+        }
+    }
+    part[(size_t)c * BP + f] = out;
+}
+
+// Held-out predictive log-likelihood of fits [0, R.B) of a finished run -> pred[f].
+template <typename Real>
+static void predictHeldOut(ResRun<Real>& R, double* pred) {
+    if (!R.teamP[0].rowFold) throw std::logic_error("held-out prediction needs row folds");
+    EngineT<Real>& E = *R.E;
+    auto& D = *E.D;
+    auto& W = *R.W;
+    hipStream_t s = E.s;
+    const int64_t K = D.K;
+    const int BP = R.BP;
+    const int m = D.model;
+    // groups (LR / PR rows, CLR strata) or Cox chunks
+    int64_t nGroups = 0;
+    int nChunks = 0;
+    std::vector<int> cs;
+    std::vector<unsigned char> flags;   // Cox: startsNew[nChunks], inner[nChunks]
+    if (m == M_COX) {
+        constexpr int64_t CH = 2048;
+        cs.push_back(0);
+        for (int64_t k = CH; k < K;) {
+            // move forward while row k continues an event run of row k - 1
+            while (k < K && D.y[k] != 0.0 && D.y[k - 1] != 0.0 && D.time[k] == D.time[k - 1] &&
+                   D.strat[k] == D.strat[k - 1])
+                ++k;
+            if (k < K) cs.push_back((int)k);
+            k += CH;
+        }
+        cs.push_back((int)K);
+        nChunks = (int)cs.size() - 1;
+        flags.assign(2 * (size_t)nChunks, 0);
+        for (int c = 0; c < nChunks; ++c) {
+            const int a = cs[c];
+            flags[c] = (c == 0 || D.strat[a] != D.strat[a - 1]) ? 1 : 0;
+            for (int k = a + 1; k < cs[c + 1]; ++k)
+                if (D.strat[k] != D.strat[k - 1]) { flags[nChunks + c] = 1; break; }
+        }
+    } else {
+        nGroups = m == M_CLR ? D.nStrata : K;
+    }
+    const int64_t nParts = m == M_COX ? nChunks : (nGroups * BP + BLOCK - 1) / BLOCK;
+    const size_t need = (size_t)nParts * BP * (m == M_COX ? 3 : 1) + (size_t)BP + (size_t)cs.size() + flags.size();
+    if (W.predCap < need) {
+        dfree(W.predBuf);
+        W.predCap = 0;
+        W.predBuf = dalloc<double>(need, W.bytes);
+        W.predCap = need;
+    }
+    double* part = W.predBuf;
+    double* outDev = part + (size_t)nParts * BP * (m == M_COX ? 3 : 1);
+    int* csDev = reinterpret_cast<int*>(outDev + BP);
+    unsigned char* flagDev = reinterpret_cast<unsigned char*>(csDev + cs.size());
+    if (m == M_COX) {
+        h2d(csDev, cs.data(), cs.size(), s);
+        h2d(flagDev, flags.data(), flags.size(), s);
+        if (!W.timeDev) {   // FP64 times, so distinct times never tie
+            W.timeDev = dalloc<double>(K, W.bytes);
+            h2d(W.timeDev, D.time.data(), K, s);
+        }
+    }
+    std::vector<double> h(BP);
+    for (int t = 0; t < R.nTeams; ++t) {
+        const res::Params<Real>& Q = R.teamP[t];
+        const int Bt = Q.B;
+        if (m == M_COX) {
+            double* tail = part + (size_t)nChunks * BP;
+            double* carry = tail + (size_t)nChunks * BP;
+            const int grid = (int)(((int64_t)nChunks * BP + BLOCK - 1) / BLOCK);
+            kPredCoxTail<Real><<<grid, BLOCK, 0, s>>>(nChunks, csDev, BP, Bt, D.dStrat, Q.xb, Q.xbLo, Q.rowFold,
+                                                      Q.held, tail);
+            kPredCoxCarry<<<1, std::max(64, BP), 0, s>>>(nChunks, BP, Bt, flagDev, flagDev + nChunks, tail, carry);
+            kPredCoxWalk<Real><<<grid, BLOCK, 0, s>>>(nChunks, csDev, BP, Bt, D.dStrat, W.timeDev, D.dY, Q.xb, Q.xbLo,
+                                                      Q.rowFold, Q.held, carry, part);
+        } else {
+            const int grid = (int)nParts;
+            switch (m) {
+            case M_LOGISTIC:
+                kPredGroups<Real, M_LOGISTIC><<<grid, BLOCK, 0, s>>>(nGroups, nullptr, BP, Bt, Q.xb, Q.xbLo, D.dY,
+                                                                     Q.rowFold, Q.held, part);
+                break;
+            case M_POISSON:
+                kPredGroups<Real, M_POISSON><<<grid, BLOCK, 0, s>>>(nGroups, nullptr, BP, Bt, Q.xb, Q.xbLo, D.dY,
+                                                                    Q.rowFold, Q.held, part);
+                break;
+            default:
+                kPredGroups<Real, M_CLR><<<grid, BLOCK, 0, s>>>(nGroups, D.dStratStart, BP, Bt, Q.xb, Q.xbLo, D.dY,
+                                                                Q.rowFold, Q.held, part);
+                break;
+            }
+        }
+        kPredSum<<<1, std::max(64, BP), 0, s>>>(nParts, BP, Bt, part, outDev);
+        CCD_HIP(hipGetLastError());
+        CCD_HIP(hipMemcpyAsync(h.data(), outDev, Bt * sizeof(double), hipMemcpyDeviceToHost, s));
+        CCD_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < Bt; ++i) pred[(size_t)t * R.teamP[0].B + i] = h[i];
+    }
+}
+
 // Run fits [0, B) of engine E as persistent launches until all have finished;
 // fills betas/flags/iters/ll.
 template <typename Real>
 static void runResident(EngineT<Real>& E, const ccd_fit_args& a, int B, const double* var, const int32_t* held,
                         const int32_t* rowFold, int F, const double* kwHost, const double* initBound, double* betas,
                         int* flags, int* iters, double* lls, double& deviceMs, double& algoBytes, double& steps,
-                        int& nLaunches, Layout L, const int32_t* fixJ = nullptr) {
+                        int& nLaunches, Layout L, const int32_t* fixJ = nullptr, double* pred = nullptr) {
     ResRun<Real> R = prepareResident<Real>(E, a, B, var, held, rowFold, F, kwHost, initBound, betas, L, fixJ);
     std::vector<ResRun<Real>*> one{&R};
     launchRuns<Real>(one, deviceMs, nLaunches);
     gatherRun<Real>(R, betas, flags, iters, lls, algoBytes, steps);
+    if (pred) predictHeldOut<Real>(R, pred);   // while the run's xb is still in HBM
 }
 
 template <typename Real>
@@ -982,7 +1217,7 @@ static void batchResults(const EngineT<Real>& E, const ccd_fit_args& a, const do
 
 template <typename Real>
 void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs, double* betas, ccd_fit_result* r,
-                             bool autotune) {
+                             bool autotune, double* predHeldOut) {
     CCD_HIP(hipSetDevice(D->device));
     const auto t0 = std::chrono::steady_clock::now();
     const int J = D->J;
@@ -1017,9 +1252,12 @@ void EngineT<Real>::fitBatch(const ccd_fit_args& a, const ccd_batch_args& bargs,
             }
             double ms = 0, bytes = 0, steps = 0;
             int nl = 0;
+            std::vector<double> pr(predHeldOut ? B : 0);
             runResident<Real>(*this, a, B, v.data(), h.data(), bargs.row_fold, F, nullptr, bd.data(), bb.data(),
                               fl.data(), it.data(), l.data(), ms, bytes, steps, nl, L,
-                              bargs.fixed_index ? fj.data() : nullptr);
+                              bargs.fixed_index ? fj.data() : nullptr, predHeldOut ? pr.data() : nullptr);
+            if (predHeldOut)
+                for (int i = 0; i < B; ++i) predHeldOut[pending[c0 + i]] = pr[i];
             if (timing) {
                 counters.deviceMs += ms;
                 counters.launches += nl;
@@ -1152,7 +1390,7 @@ public:
             a2.max_bound_count = args.max_bound_count - 1;
             ccd_batch_args b2{m, v.data(), h.data(), bargs.row_fold, fixIdx.empty() ? nullptr : fj.data()};
             std::vector<ccd_fit_result> r2(m);
-            E.fitBatch(a2, b2, bb.data(), r2.data(), false);
+            E.fitBatch(a2, b2, bb.data(), r2.data(), false, nullptr);
             for (int i = 0; i < m; ++i) {
                 const int f = retry[i];
                 std::copy(bb.begin() + (size_t)i * J, bb.begin() + (size_t)(i + 1) * J, betas.begin() + (size_t)f * J);

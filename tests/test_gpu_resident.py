@@ -211,13 +211,16 @@ def test_grouped_multi_workgroup_matches_oracle(E, model):
     assert abs(r["log_likelihood"] - ro["log_likelihood"]) <= 1e-9 * abs(ro["log_likelihood"])
 
 
-@pytest.mark.parametrize("reset", [True, False])
-@pytest.mark.parametrize("model", ["lr", "clr"])
+@pytest.mark.parametrize("model,reset", [("lr", True), ("pr", True), ("clr", True), ("cox", True),
+                                         ("lr", False), ("clr", False)])
 def test_cross_validation_matches_oracle(E, model, reset):
-    """Cold start (all fits batched) and R's default warm-start chain."""
+    """Cold start (all fits batched, every held-out predictive log-likelihood formed on
+    the device from the run's xb) for every model, and R's default warm-start chain
+    (host driver over resident fits); each fold's held-out value, the grid means, the
+    chosen variance and the final refit."""
     d = _cases.ALL[model]()
     print(f"[case] cross-validation {model} reset_coefficients={reset}", flush=True)
-    prior = create_prior("laplace", 1.0)
+    prior = create_prior("normal" if model == "pr" else "laplace", 1.0)
     ctl = create_control(convergence_type="mittal", tolerance=1e-8, fold=4, grid_size=5,
                          lower_limit=0.01, upper_limit=5.0, seed=123, reset_coefficients=reset)
     selector = _abi.SELECT_BY_PID if model == "clr" else _abi.SELECT_BY_ROW
@@ -225,10 +228,38 @@ def test_cross_validation_matches_oracle(E, model, reset):
     eng = E.Engine(d, 64)
     rg = eng.cross_validate(prior, ctl, selector)
     eng.close()
+    assert np.allclose(rg["pred"], ro["pred"], rtol=1e-8, atol=1e-10), (rg["pred"], ro["pred"])
     assert np.allclose(rg["grid"], ro["grid"], rtol=1e-8), (rg["grid"], ro["grid"])
     assert rg["best_variance"] == pytest.approx(ro["best_variance"], rel=1e-12)
     assert _rel(rg["beta"], ro["beta"]) <= 1e-6
     assert rg["n_fits"] == 20
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_cross_validation_on_the_clr_workload(E, precision):
+    """The grid-search driver on the conditional-logistic workload generator (1% scale:
+    5,000 strata, about 32,000 rows, 1,000 columns), 5 folds by stratum x 4 Laplace
+    variances, cold start, against the oracle's driver: per-fold held-out values, grid
+    means and the chosen variance (FP64 to 1e-8; FP32 bulk to 1e-6 relative, same
+    choice), final refit to 1e-6 in FP64."""
+    from cyclops_amd import data as D
+    d, _ = D.make_config(3, scale=0.01)
+    print(f"[case] cross-validation on the CLR workload K={d.K} J={d.J} fp{precision}", flush=True)
+    prior = create_prior("laplace", 0.1)
+    ctl = create_control(convergence_type="mittal", tolerance=1e-7, fold=5, grid_size=4,
+                         lower_limit=1e-3, upper_limit=1.0, seed=77, reset_coefficients=True)
+    ro = oracle.cross_validate(d, prior, ctl, _abi.SELECT_BY_PID)
+    eng = E.Engine(d, precision)
+    try:
+        rg = eng._cross_validate(prior, ctl, _abi.SELECT_BY_PID)
+    finally:
+        eng.close()
+    tol = 1e-8 if precision == 64 else 1e-6
+    assert np.allclose(rg["pred"], ro["pred"], rtol=tol), (rg["pred"], ro["pred"])
+    assert np.allclose(rg["grid"], ro["grid"], rtol=tol), (rg["grid"], ro["grid"])
+    assert rg["best_variance"] == pytest.approx(ro["best_variance"], rel=1e-12)
+    if precision == 64:
+        assert _rel(rg["beta"], ro["beta"]) <= 1e-6
 
 
 def test_fold_assignment_matches_oracle(E):
