@@ -210,6 +210,10 @@ struct ccd_server {
                 std::vector<double>().swap(r->time);
                 std::vector<int32_t>().swap(r->pid);
                 std::vector<double>().swap(r->weights);
+                std::vector<uint8_t>().swap(r->exclude);
+                std::vector<uint8_t>().swap(r->fixed);
+                r->args.exclude = r->args.fixed = nullptr;
+                r->args.weights = nullptr;
             }
             done.notify_all();
             work.notify_all();
@@ -378,6 +382,21 @@ int ccd_server_poll(ccd_server_handle s, int64_t id, int wait, double* beta_out,
         if (beta_out) std::copy(r->beta.begin(), r->beta.end(), beta_out);
         if (r->rc != CCD_OK) ccd::setLastError(r->error);
         return r->rc;
+    });
+}
+
+int ccd_server_release(ccd_server_handle s, int64_t id) {
+    return sguard([&] {
+        if (!s) throw std::invalid_argument("null server handle");
+        std::lock_guard<std::mutex> lk(s->m);
+        auto it = s->all.find(id);
+        if (it == s->all.end()) throw std::invalid_argument("unknown request id");
+        if (it->second->state != DONE) {
+            ccd::setLastError("request still running");
+            return CCD_ERR_STATE;
+        }
+        s->all.erase(it);
+        return CCD_OK;
     });
 }
 

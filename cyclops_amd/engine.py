@@ -88,6 +88,7 @@ def lib():
                                             C.POINTER(C.c_int64)]
             L.ccd_server_poll.argtypes = [vp, C.c_int64, C.c_int, dp, C.POINTER(_abi.FitResult),
                                           C.POINTER(_abi.RequestStats)]
+            L.ccd_server_release.argtypes = [vp, C.c_int64]
             L.ccd_server_stop.argtypes = [vp]
         _LIB = L
     return _LIB
@@ -553,8 +554,9 @@ class Server:
         self._J[rid.value] = data.J
         return rid.value
 
-    def poll(self, request_id, wait=True):
-        """(beta, result, stats), or None while the request runs and wait is False."""
+    def poll(self, request_id, wait=True, release=True):
+        """(beta, result, stats), or None while the request runs and wait is False.
+        With release (the default) the server forgets the request once it is read."""
         J = self._J.get(request_id, 0)
         beta = np.zeros(J)
         res = _abi.FitResult()
@@ -563,9 +565,20 @@ class Server:
                                    C.byref(st))
         if rc == _abi.ERR_STATE and not wait:
             return None
+        if release and rc in (0, _abi.ERR_HIP, _abi.ERR_INVALID, _abi.ERR_NOMEM, _abi.ERR_UNSUPPORTED):
+            err = lib().ccd_last_error().decode() if rc != 0 else ""
+            lib().ccd_server_release(self.h, request_id)
+            self._J.pop(request_id, None)
+            if rc != 0:
+                raise CyclopsError(f"ccd error {rc}: {err}")
         check(rc)
         return beta, res.as_dict(), {"submit": st.submit_time, "start": st.start_time, "end": st.end_time,
                                      "footprint_bytes": st.footprint_bytes}
+
+    def release(self, request_id):
+        """Forgets a finished request (ccd_server_release)."""
+        check(lib().ccd_server_release(self.h, request_id))
+        self._J.pop(request_id, None)
 
     def stop(self):
         if getattr(self, "h", None):

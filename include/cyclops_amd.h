@@ -373,6 +373,10 @@ int ccd_server_submit(ccd_server_handle s, const ccd_data_desc* data, const ccd_
  * request is still running and wait == 0. beta_out: [J] or NULL. */
 int ccd_server_poll(ccd_server_handle s, int64_t request_id, int wait, double* beta_out,
                     ccd_fit_result* result, ccd_request_stats* stats);
+/* Forgets a finished request (its coefficients, result and timestamps); its id is
+ * unknown afterwards. CCD_ERR_STATE while it still runs. A request's inputs are
+ * freed when it finishes; a long-running server releases each result once read. */
+int ccd_server_release(ccd_server_handle s, int64_t request_id);
 int ccd_server_stop(ccd_server_handle s);
 
 /* Estimated HBM bytes ccd_create() allocates for `data` (admission control). */

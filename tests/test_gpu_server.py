@@ -6,6 +6,7 @@ import pytest
 
 import _cases
 import oracle
+from cyclops_amd import _abi
 from cyclops_amd import data as D
 from cyclops_amd.control import create_control, create_prior
 
@@ -155,8 +156,28 @@ def test_over_budget_request_is_refused(E):
 def test_poll_without_wait(E):
     srv = E.Server(slots=1, hbm_budget_bytes=8e9)
     rid = srv.submit(_cases.logistic(K=500, seed=4), create_prior("laplace", 1.0), create_control())
-    out = srv.poll(rid, wait=False)
+    out = srv.poll(rid, wait=False, release=False)
     assert out is None or out[1]["return_flag"] == 0
     b, r, st = srv.poll(rid, wait=True)
     assert r["return_flag"] == 0 and st["end"] >= st["start"] >= st["submit"]
+    srv.stop()
+
+
+def test_finished_requests_are_released(E):
+    """ccd_server_release forgets a finished request: polling it afterwards is an unknown
+    id; a request that still runs is refused with CCD_ERR_STATE."""
+    import ctypes as C
+    srv = E.Server(slots=1, hbm_budget_bytes=8e9)
+    L = E.lib()
+    long_ctl = create_control(convergence_type="mittal", tolerance=1e-12, max_iterations=400)
+    slow = srv.submit(_cases.poisson(K=60_000, seed=8), create_prior("normal", 1.0), long_ctl)
+    quick = srv.submit(_cases.logistic(K=500, seed=5), create_prior("laplace", 1.0), create_control())
+    rc = L.ccd_server_release(srv.h, C.c_int64(quick))   # queued behind the slow one (one slot)
+    assert rc in (_abi.ERR_STATE, 0)
+    if rc == _abi.ERR_STATE:
+        b, r, _ = srv.poll(quick, wait=True)   # reads and releases
+        assert r["return_flag"] == 0
+    with pytest.raises(E.CyclopsError, match="unknown request id"):
+        srv.poll(quick, wait=True)
+    srv.poll(slow, wait=True)
     srv.stop()
