@@ -169,6 +169,7 @@ struct ResWork {
     int* pid = nullptr;       // Cox: unweighted tie pids
     int* pstart = nullptr;
     int* pseg = nullptr;
+    unsigned char* pflag = nullptr;   // Cox: 1 where a pid begins a stratum
     // per batch
     Real* xb = nullptr;
     unsigned short* xbLo = nullptr;   // FP32 runs: low part of xb (bfloat16 bits)
@@ -210,6 +211,7 @@ struct ResWork {
     void freeAll() {
         freeBatch();
         dfree(rowStart); dfree(pidStart); dfree(colOff); dfree(rng); dfree(pid); dfree(pstart); dfree(pseg);
+        dfree(pflag);
         G = 0;
     }
 };
@@ -396,6 +398,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
         W.freeAll();
         W.G = G;
         std::vector<int32_t> pid(K), pstart, pseg;
+        std::vector<unsigned char> fl;
         int64_t N;
         if (m == M_COX) {   // unweighted Breslow tie groups (ModelSpecifics.hpp:1809-1868 with no weights)
             int cur = 0;
@@ -458,6 +461,12 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
             W.pseg = dalloc<int>(N + 1, Bb);
             pseg.push_back(-1);
             h2d(W.pseg, pseg.data(), N + 1, s);
+            if (m == M_COX) {   // fl lives until the synchronisation below
+                fl.assign(N, 0);
+                for (int64_t p = 0; p < N; ++p) fl[p] = (p == 0 || pseg[p] != pseg[p - 1]) ? 1 : 0;
+                W.pflag = dalloc<unsigned char>(N, Bb);
+                h2d(W.pflag, fl.data(), N, s);
+            }
         }
         CCD_HIP(hipStreamSynchronize(s));
     }
@@ -620,7 +629,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     P.dense = D.dDense; P.y = D.dY;
     P.seg = D.dStrat;
     P.pid = (m == M_COX) ? W.pid : D.dStrat;
-    P.pstart = W.pstart; P.pseg = W.pseg;
+    P.pstart = W.pstart; P.pseg = W.pseg; P.pflag = W.pflag;
     P.rowStart = W.rowStart; P.pidStart = W.pidStart; P.colOff = W.colOff;
     P.info = W.info; P.rng = W.rng; P.firstJ = firstJ;
     P.kw = kwHost ? W.kw : nullptr;

@@ -56,6 +56,7 @@ struct Params {
     const int* pid;       // pid per row (CLR: stratum, Cox: tie group)
     const int* pstart;    // [N+1] first row of each pid
     const int* pseg;      // [N] stratum of each pid
+    const unsigned char* pflag;   // [N] 1 where a pid begins a stratum (Cox)
     const int* rowStart;  // [G+1]
     const int* pidStart;  // [G+1]
     const int* colOff;    // [J*(G+1)] entry offset of workgroup w inside column j (INDICATOR/SPARSE)
@@ -962,29 +963,36 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
         if (lane == 0) excl = z;
         all = bcast63(inc);
     };
-    // walks [a, b) forward: D = segmented prefix of denom; a pid whose stratum
-    // differs from the previous pid's starts a new segment
-    auto fwdWalk = [&](Seg2 run, bool store) {
+    // Segment starts come from one byte per pid (pflag[p] = pid p begins a stratum):
+    // forward, p > 0 with pflag[p]; backward (descending pids), p == N - 1 or
+    // pflag[p + 1].
+    const unsigned char* fl = P.pflag;
+    // walks [a, b) forward: D = segmented prefix of denom. With store it writes D and
+    // also forms, from those D, the wave's total for the backward scan (its pids in
+    // descending order: each lane's pids from the top, then the lanes from the top),
+    // so the backward totals need no walk of their own.
+    auto fwdWalk = [&](Seg2 run, bool store, Seg2& bwdTot) {
         for (int64_t base = a; base < b; base += CW) {
             const int64_t p0 = base + (int64_t)lane * CH;
-            double dv[CH];
-            int sg[CH];
-            const int sPrev = (p0 > 0 && p0 < b) ? P.pseg[p0 - 1] : 0;
+            double dv[CH], nv[CH];
+            bool st[CH], stB[CH];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                const bool ok = p0 + i < b;
-                dv[i] = ok ? P.denom[p0 + i] : 0.0;
-                sg[i] = ok ? P.pseg[p0 + i] : 0;
+                const int64_t p = p0 + i;
+                const bool ok = p < b;
+                dv[i] = ok ? P.denom[p] : 0.0;
+                st[i] = ok && p > 0 && fl[p] != 0;
+                if (store) {
+                    nv[i] = ok ? P.nW[p] : 0.0;
+                    stB[i] = ok && (p == N - 1 || fl[p + 1] != 0);
+                }
             }
             Seg2 acc = z;
             double inc[CH];
             bool started[CH];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                const int64_t p = p0 + i;
-                const int prev = i == 0 ? sPrev : sg[i - 1];
-                const bool st = p < b && p > 0 && sg[i] != prev;
-                acc = segCombine(acc, Seg2{dv[i], 0.0, st ? 1 : 0, 0});
+                acc = segCombine(acc, Seg2{dv[i], 0.0, st[i] ? 1 : 0, 0});
                 inc[i] = acc.a;
                 started[i] = acc.f != 0;
             }
@@ -992,41 +1000,56 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
             laneJoin(acc, excl, all);
             if (store) {
                 const Seg2 pre = segCombine(run, excl);
+                double dd[CH];
 #pragma unroll
-                for (int i = 0; i < CH; ++i)
-                    if (p0 + i < b) P.Dacc[p0 + i] = started[i] ? inc[i] : pre.a + inc[i];
+                for (int i = 0; i < CH; ++i) {
+                    dd[i] = started[i] ? inc[i] : pre.a + inc[i];
+                    if (p0 + i < b) P.Dacc[p0 + i] = dd[i];
+                }
+                Seg2 bacc = z;   // this lane's pids, descending
+#pragma unroll
+                for (int i = CH - 1; i >= 0; --i) {
+                    if (p0 + i >= b) continue;
+                    const double r = nv[i] == 0.0 ? 0.0 : nv[i] / dd[i];
+                    bacc = segCombine(bacc, Seg2{r, nv[i] == 0.0 ? 0.0 : r / dd[i], stB[i] ? 1 : 0, 0});
+                }
+                // lanes from the top: an inclusive scan over reversed lanes, total at lane 63
+                Seg2 rv;
+                rv.a = __shfl(bacc.a, 63 - lane, 64);
+                rv.b = __shfl(bacc.b, 63 - lane, 64);
+                rv.f = __shfl(bacc.f, 63 - lane, 64);
+                rv.pad = 0;
+                const Seg2 chunk = bcast63(waveSegScanIncl(rv));
+                bwdTot = segCombine(chunk, bwdTot);   // this chunk's pids precede the earlier ones
             }
             run = segCombine(run, all);
         }
         return run;
     };
     // walks [a, b) backward (descending pids): R1, R2 = reverse segmented sums of
-    // nE/D and nE/D^2; the last pid of a stratum (or of all pids) starts a segment
-    auto bwdWalk = [&](Seg2 run, bool store) {
+    // nE/D and nE/D^2 from the carry `run`
+    auto bwdWalk = [&](Seg2 run) {
         for (int64_t top = b - 1; top >= a; top -= CW) {
             const int64_t p0 = top - (int64_t)lane * CH;   // this lane's pids: p0, p0 - 1, ...
             double nv[CH], dv[CH];
-            int sg[CH];
-            const int sNext = (p0 >= a && p0 + 1 < N) ? P.pseg[p0 + 1] : 0;
+            bool stB[CH];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
                 const int64_t p = p0 - i;
                 const bool ok = p >= a;
                 nv[i] = ok ? P.nW[p] : 0.0;
                 dv[i] = ok ? P.Dacc[p] : 1.0;
-                sg[i] = ok ? P.pseg[p] : 0;
+                stB[i] = ok && (p == N - 1 || fl[p + 1] != 0);
             }
             Seg2 acc = z;
             double i1[CH], i2[CH];
             bool started[CH];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                const int64_t p = p0 - i;
                 Seg2 v = z;
-                if (p >= a) {
-                    const int next = i == 0 ? sNext : sg[i - 1];
+                if (p0 - i >= a) {
                     const double r = nv[i] == 0.0 ? 0.0 : nv[i] / dv[i];
-                    v = Seg2{r, nv[i] == 0.0 ? 0.0 : r / dv[i], (p == N - 1 || next != sg[i]) ? 1 : 0, 0};
+                    v = Seg2{r, nv[i] == 0.0 ? 0.0 : r / dv[i], stB[i] ? 1 : 0, 0};
                 }
                 acc = segCombine(acc, v);
                 i1[i] = acc.a;
@@ -1035,15 +1058,13 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
             }
             Seg2 excl, all;
             laneJoin(acc, excl, all);
-            if (store) {
-                const Seg2 pre = segCombine(run, excl);
+            const Seg2 pre = segCombine(run, excl);
 #pragma unroll
-                for (int i = 0; i < CH; ++i) {
-                    const int64_t p = p0 - i;
-                    if (p >= a) {
-                        P.R1[p] = started[i] ? i1[i] : pre.a + i1[i];
-                        P.R2[p] = started[i] ? i2[i] : pre.b + i2[i];
-                    }
+            for (int i = 0; i < CH; ++i) {
+                const int64_t p = p0 - i;
+                if (p >= a) {
+                    P.R1[p] = started[i] ? i1[i] : pre.a + i1[i];
+                    P.R2[p] = started[i] ? i2[i] : pre.b + i2[i];
                 }
             }
             run = segCombine(run, all);
@@ -1069,24 +1090,26 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
         }, segc, s.u.seg, z);
         return true;
     };
-    // ---- forward: D = segmented prefix of denom
-    Seg2 mine = fwdWalk(z, false);
+    // ---- forward: D = segmented prefix of denom (a totals walk, then the store walk,
+    // which also forms the backward totals)
+    Seg2 none = z;
+    Seg2 mine = fwdWalk(z, false, none);
     if (lane == 0) s.scan[wave] = mine;
     __syncthreads();
     Seg2 carry;
     if (!exchange(false, carry)) return false;
     Seg2 run = Seg2{carry.a, 0.0, 0, 0};
     for (int i = 0; i < wave; ++i) run = segCombine(run, s.scan[i]);
-    fwdWalk(run, true);
+    __syncthreads();   // every wave has read the forward totals before they are replaced
+    Seg2 back = z;
+    fwdWalk(run, true, back);
+    if (lane == 0) s.scan[wave] = back;
     __syncthreads();
     // ---- backward: R1, R2 = reverse segmented sums of nE/D, nE/D^2
-    mine = bwdWalk(z, false);
-    if (lane == 0) s.scan[wave] = mine;
-    __syncthreads();
     if (!exchange(true, carry)) return false;
     run = Seg2{carry.a, carry.b, 0, 0};
     for (int i = WAVES - 1; i > wave; --i) run = segCombine(run, s.scan[i]);
-    bwdWalk(run, true);
+    bwdWalk(run);
     __syncthreads();
     return true;
 }
