@@ -86,6 +86,54 @@ __device__ __forceinline__ Seg2 waveSegScanIncl(Seg2 v) {
     return v;
 }
 
+// The same scan on DPP lane moves (CDNA's row shifts and row broadcasts), which
+// stay in the VALU instead of a round trip through the LDS crossbar per step:
+// Hillis-Steele inside each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's
+// total into row 1 and row 2's into row 3 (row_bcast:15), then the total of rows
+// 0-1 into rows 2-3 (row_bcast:31). Lanes whose source is outside the row or
+// masked keep the identity. REV combines in descending lane order (lane 63 then
+// holds x63 o x62 o ... o x0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dppInt(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dppDouble(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = dppInt<CTRL, ROWS>((int)(u & 0xffffffffll));
+    const int hi = dppInt<CTRL, ROWS>((int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL, int ROWS, bool REV>
+__device__ __forceinline__ void segDppStep(Seg2& v) {
+    const Seg2 p{dppDouble<CTRL, ROWS>(v.a), dppDouble<CTRL, ROWS>(v.b), dppInt<CTRL, ROWS>(v.f), 0};
+    v = REV ? segCombine(v, p) : segCombine(p, v);
+}
+template <bool REV = false>
+__device__ __forceinline__ Seg2 waveSegScanDpp(Seg2 v) {
+    segDppStep<0x111, 0xf, REV>(v);   // row_shr:1
+    segDppStep<0x112, 0xf, REV>(v);   // row_shr:2
+    segDppStep<0x114, 0xf, REV>(v);   // row_shr:4
+    segDppStep<0x118, 0xf, REV>(v);   // row_shr:8
+    segDppStep<0x142, 0xa, REV>(v);   // row_bcast:15 into rows 1 and 3
+    segDppStep<0x143, 0xc, REV>(v);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+// value of the previous lane (identity at lane 0): wave_shr:1
+__device__ __forceinline__ Seg2 segFromPrevLane(const Seg2& v) {
+    return Seg2{dppDouble<0x138, 0xf>(v.a), dppDouble<0x138, 0xf>(v.b), dppInt<0x138, 0xf>(v.f), 0};
+}
+// lane 63's value in every lane (through scalar registers)
+__device__ __forceinline__ double readLane63(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ Seg2 segLane63(const Seg2& v) {
+    return Seg2{readLane63(v.a), readLane63(v.b), __builtin_amdgcn_readlane(v.f, 63), 0};
+}
+
 // Exclusive segmented scan across the workgroup; `total` receives the
 // combination of all threads' values. Ends with a barrier (safe to call again).
 __device__ __forceinline__ Seg2 blockSegScanExcl(Seg2 v, Seg2& total) {

@@ -940,59 +940,60 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
     const int64_t N = P.N;
     const Seg2 z{0.0, 0.0, 0, 0};
     auto segc = [](const Seg2& x, const Seg2& y) { return segCombine(x, y); };
-    auto bcast63 = [](const Seg2& v) {
-        Seg2 r;
-        r.a = __shfl(v.a, 63, 64);
-        r.b = __shfl(v.b, 63, 64);
-        r.f = __shfl(v.f, 63, 64);
-        r.pad = 0;
-        return r;
-    };
     // Chunks of CH consecutive pids per lane (64 * CH per wave): each lane scans its
     // pids in registers and ONE wave scan of the lane totals joins them, instead of
     // a wave scan per 64 pids.
     constexpr int CH = 8;
     constexpr int CW = 64 * CH;
-    // wave exclusive prefix of the lane totals and the wave's total
+    // wave exclusive prefix of the lane totals and the wave's total (DPP scan)
     auto laneJoin = [&](const Seg2& tot, Seg2& excl, Seg2& all) {
-        const Seg2 inc = waveSegScanIncl(tot);
-        excl.a = __shfl_up(inc.a, 1, 64);
-        excl.b = __shfl_up(inc.b, 1, 64);
-        excl.f = __shfl_up(inc.f, 1, 64);
-        excl.pad = 0;
-        if (lane == 0) excl = z;
-        all = bcast63(inc);
+        const Seg2 inc = waveSegScanDpp<false>(tot);
+        excl = segFromPrevLane(inc);
+        all = segLane63(inc);
     };
     // Segment starts come from one byte per pid (pflag[p] = pid p begins a stratum):
     // forward, p > 0 with pflag[p]; backward (descending pids), p == N - 1 or
     // pflag[p + 1].
     const unsigned char* fl = P.pflag;
+    // A walk keeps one chunk of loads in flight: the loads of chunk c + 1 are issued
+    // (at addresses clamped into the range, masked later) before chunk c is scanned,
+    // so a wave waits for memory once per walk rather than once per chunk.
+    struct Raw {
+        double x[CH], y[CH];
+        int f[CH + 1];
+    };
+    // forward chunk at base: x = denom, y = nW (store walk), f[i] = pflag[p0 + i]
+    auto fetchFwd = [&](int64_t base, bool store, Raw& r) {
+        const int64_t p0 = base + (int64_t)lane * CH;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int64_t p = p0 + i < b ? p0 + i : b - 1;
+            r.x[i] = P.denom[p];
+            if (store) r.y[i] = P.nW[p];
+        }
+#pragma unroll
+        for (int i = 0; i <= CH; ++i) r.f[i] = fl[p0 + i < N ? p0 + i : N - 1];
+    };
     // walks [a, b) forward: D = segmented prefix of denom. With store it writes D and
     // also forms, from those D, the wave's total for the backward scan (its pids in
     // descending order: each lane's pids from the top, then the lanes from the top),
     // so the backward totals need no walk of their own.
     auto fwdWalk = [&](Seg2 run, bool store, Seg2& bwdTot) {
+        Raw nxt;
+        if (a < b) fetchFwd(a, store, nxt);
         for (int64_t base = a; base < b; base += CW) {
+            const Raw cur = nxt;
+            if (base + CW < b) fetchFwd(base + CW, store, nxt);
             const int64_t p0 = base + (int64_t)lane * CH;
-            double dv[CH], nv[CH];
-            bool st[CH], stB[CH];
-#pragma unroll
-            for (int i = 0; i < CH; ++i) {
-                const int64_t p = p0 + i;
-                const bool ok = p < b;
-                dv[i] = ok ? P.denom[p] : 0.0;
-                st[i] = ok && p > 0 && fl[p] != 0;
-                if (store) {
-                    nv[i] = ok ? P.nW[p] : 0.0;
-                    stB[i] = ok && (p == N - 1 || fl[p + 1] != 0);
-                }
-            }
             Seg2 acc = z;
             double inc[CH];
             bool started[CH];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                acc = segCombine(acc, Seg2{dv[i], 0.0, st[i] ? 1 : 0, 0});
+                const int64_t p = p0 + i;
+                const bool ok = p < b;
+                const bool st = ok && p > 0 && cur.f[i] != 0;
+                acc = segCombine(acc, Seg2{ok ? cur.x[i] : 0.0, 0.0, st ? 1 : 0, 0});
                 inc[i] = acc.a;
                 started[i] = acc.f != 0;
             }
@@ -1009,47 +1010,55 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
                 Seg2 bacc = z;   // this lane's pids, descending
 #pragma unroll
                 for (int i = CH - 1; i >= 0; --i) {
-                    if (p0 + i >= b) continue;
-                    const double r = nv[i] == 0.0 ? 0.0 : nv[i] / dd[i];
-                    bacc = segCombine(bacc, Seg2{r, nv[i] == 0.0 ? 0.0 : r / dd[i], stB[i] ? 1 : 0, 0});
+                    const int64_t p = p0 + i;
+                    if (p >= b) continue;
+                    const bool stE = p == N - 1 || cur.f[i + 1] != 0;
+                    const double nv = cur.y[i];
+                    // one divide: nE/D, then nE/D^2 from it (D may be 0 where nE is)
+                    const double inv = nv == 0.0 ? 0.0 : 1.0 / dd[i];
+                    const double r = nv * inv;
+                    bacc = segCombine(bacc, Seg2{r, r * inv, stE ? 1 : 0, 0});
                 }
-                // lanes from the top: an inclusive scan over reversed lanes, total at lane 63
-                Seg2 rv;
-                rv.a = __shfl(bacc.a, 63 - lane, 64);
-                rv.b = __shfl(bacc.b, 63 - lane, 64);
-                rv.f = __shfl(bacc.f, 63 - lane, 64);
-                rv.pad = 0;
-                const Seg2 chunk = bcast63(waveSegScanIncl(rv));
-                bwdTot = segCombine(chunk, bwdTot);   // this chunk's pids precede the earlier ones
+                // lanes from the top: a scan in descending lane order, total at lane 63
+                bwdTot = segCombine(segLane63(waveSegScanDpp<true>(bacc)), bwdTot);   // precedes the earlier chunks
             }
             run = segCombine(run, all);
         }
         return run;
     };
+    // backward chunk at top: x = nW, y = D, f[i] = pflag[p + 1] of this lane's pid p0 - i
+    auto fetchBwd = [&](int64_t top, Raw& r) {
+        const int64_t p0 = top - (int64_t)lane * CH;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int64_t p = p0 - i >= a ? p0 - i : a;
+            r.x[i] = P.nW[p];
+            r.y[i] = P.Dacc[p];
+            r.f[i] = fl[p + 1 < N ? p + 1 : N - 1];
+        }
+    };
     // walks [a, b) backward (descending pids): R1, R2 = reverse segmented sums of
     // nE/D and nE/D^2 from the carry `run`
     auto bwdWalk = [&](Seg2 run) {
+        Raw nxt;
+        if (a < b) fetchBwd(b - 1, nxt);
         for (int64_t top = b - 1; top >= a; top -= CW) {
+            const Raw cur = nxt;
+            if (top - CW >= a) fetchBwd(top - CW, nxt);
             const int64_t p0 = top - (int64_t)lane * CH;   // this lane's pids: p0, p0 - 1, ...
-            double nv[CH], dv[CH];
-            bool stB[CH];
-#pragma unroll
-            for (int i = 0; i < CH; ++i) {
-                const int64_t p = p0 - i;
-                const bool ok = p >= a;
-                nv[i] = ok ? P.nW[p] : 0.0;
-                dv[i] = ok ? P.Dacc[p] : 1.0;
-                stB[i] = ok && (p == N - 1 || fl[p + 1] != 0);
-            }
             Seg2 acc = z;
             double i1[CH], i2[CH];
             bool started[CH];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
+                const int64_t p = p0 - i;
                 Seg2 v = z;
-                if (p0 - i >= a) {
-                    const double r = nv[i] == 0.0 ? 0.0 : nv[i] / dv[i];
-                    v = Seg2{r, nv[i] == 0.0 ? 0.0 : r / dv[i], stB[i] ? 1 : 0, 0};
+                if (p >= a) {
+                    const double nv = cur.x[i], dv = cur.y[i];
+                    const bool stE = p == N - 1 || cur.f[i] != 0;
+                    const double inv = nv == 0.0 ? 0.0 : 1.0 / dv;
+                    const double r = nv * inv;
+                    v = Seg2{r, r * inv, stE ? 1 : 0, 0};
                 }
                 acc = segCombine(acc, v);
                 i1[i] = acc.a;
@@ -1194,8 +1203,9 @@ __device__ __forceinline__ bool coxRescan(const Params<Real>& P, unsigned& seq, 
             for (int t = 0; t < TL; ++t) {
                 const int64_t q = p - t;
                 if (q < a) break;
-                const double r = nv[t] == 0.0 ? 0.0 : nv[t] / dv[t];
-                visit(q, Seg2{r, nv[t] == 0.0 ? 0.0 : r / dv[t], (q == N - 1 || sv[t] != sv[t + 1]) ? 1 : 0, 0});
+                const double inv = nv[t] == 0.0 ? 0.0 : 1.0 / dv[t];
+                const double r = nv[t] * inv;
+                visit(q, Seg2{r, r * inv, (q == N - 1 || sv[t] != sv[t + 1]) ? 1 : 0, 0});
             }
         }
     };
@@ -1430,6 +1440,15 @@ __device__ __forceinline__ Meta loadMeta(const Params<Real>& P, int j, int w) {
     return m;
 }
 
+// The rescan for the team's fit layout (the wave-range walk holds LDS staging, so
+// only the one-fit layout instantiates it).
+template <typename Real, int BP>
+__device__ __forceinline__ bool coxRescanAny(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
+                                             Shared<BP>& s) {
+    if constexpr (BP == 1) return coxRescanWaves<Real, BP>(P, seq, deadline, s);
+    else return coxRescan<Real, BP>(P, seq, deadline, s);
+}
+
 // The persistent findMode kernel (CyclicCoordinateDescent.cpp:909-1216 for every fit).
 // Launch = teams of T workgroups; team t runs the fits described by teams[t]
 // (its own data and state pointers), exchanging only inside the team.
@@ -1465,7 +1484,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             initPids<Real, MODEL, BP>(P);
             if constexpr (MODEL == M_COX) {
                 // the pass above wrote only this workgroup's pids; the scan reads them all
-                if (!(BP == 1 ? coxRescanWaves<Real, BP>(P, seq, deadline, s) : coxRescan<Real, BP>(P, seq, deadline, s))) return;
+                if (!coxRescanAny<Real, BP>(P, seq, deadline, s)) return;
             }
         }
     }
@@ -1689,7 +1708,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     }
                     __syncthreads();
                     if constexpr (MODEL == M_COX) {
-                        if (!(BP == 1 ? coxRescanWaves<Real, BP>(P, seq, deadline, s) : coxRescan<Real, BP>(P, seq, deadline, s))) return;
+                        if (!coxRescanAny<Real, BP>(P, seq, deadline, s)) return;
                     }
                 }
                 if (w == 0 && threadIdx.x == 0) {
