@@ -527,9 +527,9 @@ def run_other(args):
         folds, n_var = 4, 16
         lo, hi = 0.01, 1.0
     elif idx == 2:
-        prec, ptype, centre, name = _abi.FP32, "laplace", 0.05, "cox_breslow_K2e6_P5e4_density0.005_plus10dense_laplace0.05"
-        folds, n_var = 4, 1
-        lo, hi = 0.05, 0.05
+        prec, ptype, centre, name = _abi.FP32, "laplace", 0.05, "cox_breslow_K2e6_P5e4_density0.005_plus10dense_laplace0.0125-0.2"
+        folds, n_var = 4, 4
+        lo, hi = 0.0125, 0.2
     else:
         prec, ptype, centre, name = _abi.FP32, "laplace", None, "clr_cv_5e5strata_P1e5_density0.003_10fold_x_20var"
         folds, n_var = 10, 20

@@ -134,6 +134,49 @@ __device__ __forceinline__ Seg2 segLane63(const Seg2& v) {
     return Seg2{readLane63(v.a), readLane63(v.b), __builtin_amdgcn_readlane(v.f, 63), 0};
 }
 
+// Plain sum over the 64 lanes on the same DPP sequence; every lane gets the total.
+__device__ __forceinline__ double waveSumDpp(double v) {
+    v += dppDouble<0x111, 0xf>(v);
+    v += dppDouble<0x112, 0xf>(v);
+    v += dppDouble<0x114, 0xf>(v);
+    v += dppDouble<0x118, 0xf>(v);
+    v += dppDouble<0x142, 0xa>(v);
+    v += dppDouble<0x143, 0xc>(v);
+    return readLane63(v);
+}
+
+// Segmented sums of three values (risk-set denominator and the two column
+// numerators of the Breslow sweep) with one stratum-start flag.
+struct Seg3 {
+    double d, n1, n2;
+    int f;
+};
+__device__ __forceinline__ Seg3 seg3Combine(const Seg3& x, const Seg3& y) {
+    return Seg3{y.f ? y.d : x.d + y.d, y.f ? y.n1 : x.n1 + y.n1, y.f ? y.n2 : x.n2 + y.n2, x.f | y.f};
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg3DppStep(Seg3& v) {
+    const Seg3 p{dppDouble<CTRL, ROWS>(v.d), dppDouble<CTRL, ROWS>(v.n1), dppDouble<CTRL, ROWS>(v.n2),
+                 dppInt<CTRL, ROWS>(v.f)};
+    v = seg3Combine(p, v);
+}
+__device__ __forceinline__ Seg3 waveSeg3ScanDpp(Seg3 v) {
+    seg3DppStep<0x111, 0xf>(v);
+    seg3DppStep<0x112, 0xf>(v);
+    seg3DppStep<0x114, 0xf>(v);
+    seg3DppStep<0x118, 0xf>(v);
+    seg3DppStep<0x142, 0xa>(v);
+    seg3DppStep<0x143, 0xc>(v);
+    return v;
+}
+__device__ __forceinline__ Seg3 seg3FromPrevLane(const Seg3& v) {
+    return Seg3{dppDouble<0x138, 0xf>(v.d), dppDouble<0x138, 0xf>(v.n1), dppDouble<0x138, 0xf>(v.n2),
+                dppInt<0x138, 0xf>(v.f)};
+}
+__device__ __forceinline__ Seg3 seg3Lane63(const Seg3& v) {
+    return Seg3{readLane63(v.d), readLane63(v.n1), readLane63(v.n2), __builtin_amdgcn_readlane(v.f, 63)};
+}
+
 // Exclusive segmented scan across the workgroup; `total` receives the
 // combination of all threads' values. Ends with a barrier (safe to call again).
 __device__ __forceinline__ Seg2 blockSegScanExcl(Seg2 v, Seg2& total) {

@@ -170,6 +170,7 @@ struct ResWork {
     int* pstart = nullptr;
     int* pseg = nullptr;
     unsigned char* pflag = nullptr;   // Cox: 1 where a pid begins a stratum
+    int* waveLast = nullptr;          // Cox: last stratum start of each wave range [G][WAVES]
     // per batch
     Real* xb = nullptr;
     unsigned short* xbLo = nullptr;   // FP32 runs: low part of xb (bfloat16 bits)
@@ -177,6 +178,7 @@ struct ResWork {
     size_t predCap = 0;
     double* timeDev = nullptr;        // Cox survival times (held-out risk sets)
     double *denom = nullptr, *Dacc = nullptr, *R1 = nullptr, *R2 = nullptr, *nW = nullptr;
+    double *numer = nullptr, *numer2 = nullptr;   // Cox sweep (one fit per team)
     double *beta = nullptr, *hDelta = nullptr, *var = nullptr, *initBound = nullptr;
     unsigned char *exclude = nullptr, *fixed = nullptr;
     double *xjyTot = nullptr, *xjyFold = nullptr, *xjyW = nullptr, *fixedTerm = nullptr;
@@ -200,6 +202,7 @@ struct ResWork {
 
     void freeBatch() {
         dfree(xb); dfree(xbLo); dfree(predBuf); predCap = 0; dfree(timeDev); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
+        dfree(numer); dfree(numer2);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
@@ -211,7 +214,7 @@ struct ResWork {
     void freeAll() {
         freeBatch();
         dfree(rowStart); dfree(pidStart); dfree(colOff); dfree(rng); dfree(pid); dfree(pstart); dfree(pseg);
-        dfree(pflag);
+        dfree(pflag); dfree(waveLast);
         G = 0;
     }
 };
@@ -337,8 +340,15 @@ static std::vector<Layout> layoutCandidates(int model, int64_t K, int B, int dev
         out.push_back(L);
     };
     const bool grouped = model == M_CLR || model == M_COX;
+    // Cox: one fit per team, as many workgroups per team as keep every team resident
+    // (each team streams its own fit's per-pid state once per coordinate in the
+    // accumulated sweep); these probes are slow, so the first one usually decides
+    if (model == M_COX) {
+        int t = cus * occupancyOf<Real>(model, 1) / std::max(B, 1);
+        if (t >= 1) add(1, std::min(t, 512));
+    }
     // a few grouped fits: one fit per team first (each team walks its own risk sets
-    // with the one-fit rescan); these probes are slow, so the first one usually decides
+    // with the one-fit rescan)
     if (grouped && B <= 16)
         for (int t : {64, 128})
             if (K / t >= 2048) add(1, t);
@@ -399,6 +409,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
         W.G = G;
         std::vector<int32_t> pid(K), pstart, pseg;
         std::vector<unsigned char> fl;
+        std::vector<int> wl;
         int64_t N;
         if (m == M_COX) {   // unweighted Breslow tie groups (ModelSpecifics.hpp:1809-1868 with no weights)
             int cur = 0;
@@ -422,11 +433,17 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
             N = K;
         }
         W.N = N;
-        // row ranges: even split, moved to the next pid start for grouped models
+        // row ranges: even split, moved to the next pid start for grouped models;
+        // Cox splits the pids evenly instead (every coordinate walks every pid, while
+        // a column touches a few rows of each workgroup)
         std::vector<int32_t> rs(G + 1), ps(G + 1, 0);
         for (int w = 0; w <= G; ++w) {
             int64_t r = (K * w) / G;
-            if (grouped && w > 0 && w < G) {
+            if (m == M_COX && w > 0 && w < G) {
+                const int64_t p = (N * w) / G;
+                r = pstart[p];
+                ps[w] = (int32_t)p;
+            } else if (grouped && w > 0 && w < G) {
                 int64_t p = (m == M_CLR) ? ((r < K) ? D.strat[r] : N) : ((r < K) ? pid[r] : N);
                 if (p < N && pstart[p] < r) ++p;
                 r = (p < N) ? pstart[p] : K;
@@ -466,6 +483,18 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
                 for (int64_t p = 0; p < N; ++p) fl[p] = (p == 0 || pseg[p] != pseg[p - 1]) ? 1 : 0;
                 W.pflag = dalloc<unsigned char>(N, Bb);
                 h2d(W.pflag, fl.data(), N, s);
+                // the sweep's wave ranges split each workgroup's pids as the rescan does
+                wl.assign((size_t)G * WAVES, -1);
+                for (int w = 0; w < G; ++w) {
+                    const int64_t q0 = ps[w], q1 = ps[w + 1], cw = (q1 - q0 + WAVES - 1) / WAVES;
+                    for (int i = 0; i < WAVES; ++i) {
+                        const int64_t a = std::min(q0 + i * cw, q1), b = std::min(a + cw, q1);
+                        for (int64_t p = b - 1; p >= a; --p)
+                            if (p > 0 && fl[p]) { wl[(size_t)w * WAVES + i] = (int)p; break; }
+                    }
+                }
+                W.waveLast = dalloc<int>((size_t)G * WAVES, Bb);
+                h2d(W.waveLast, wl.data(), (size_t)G * WAVES, s);
             }
         }
         CCD_HIP(hipStreamSynchronize(s));
@@ -487,6 +516,10 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
             W.R1 = dalloc<double>(pn, Bb);
             W.R2 = dalloc<double>(pn, Bb);
             W.nW = dalloc<double>(pn, Bb);
+            if (m == M_COX && BP == 1) {
+                W.numer = dalloc<double>(pn, Bb);
+                W.numer2 = dalloc<double>(pn, Bb);
+            }
         }
         W.beta = dalloc<double>((size_t)B * J, Bb);
         W.hDelta = dalloc<double>((size_t)B * J, Bb);
@@ -588,6 +621,10 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     // xb = X beta for every fit (column order, as CyclicCoordinateDescent::computeXBeta);
     // team t's block is xb[t*K*BP ...], fit-minor inside the team
     CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)nTeams * K * BP * sizeof(Real), s));
+    if (W.numer) {   // the sweep keeps the numerators at zero between coordinates
+        CCD_HIP(hipMemsetAsync(W.numer, 0, (size_t)nTeams * N * sizeof(double), s));
+        CCD_HIP(hipMemsetAsync(W.numer2, 0, (size_t)nTeams * N * sizeof(double), s));
+    }
     if (W.xbLo) CCD_HIP(hipMemsetAsync(W.xbLo, 0, (size_t)nTeams * K * BP * sizeof(unsigned short), s));
     {
         // warm starts with many non-zero sparse columns form the sparse part of xb in
@@ -629,7 +666,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     P.dense = D.dDense; P.y = D.dY;
     P.seg = D.dStrat;
     P.pid = (m == M_COX) ? W.pid : D.dStrat;
-    P.pstart = W.pstart; P.pseg = W.pseg; P.pflag = W.pflag;
+    P.pstart = W.pstart; P.pseg = W.pseg; P.pflag = W.pflag; P.waveLast = W.waveLast;
     P.rowStart = W.rowStart; P.pidStart = W.pidStart; P.colOff = W.colOff;
     P.info = W.info; P.rng = W.rng; P.firstJ = firstJ;
     P.kw = kwHost ? W.kw : nullptr;
@@ -654,6 +691,8 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
         if (grouped) {
             const size_t o = (size_t)t * N * BP;
             Q.denom = W.denom + o; Q.Dacc = W.Dacc + o; Q.R1 = W.R1 + o; Q.R2 = W.R2 + o; Q.nW = W.nW + o;
+            Q.numer = W.numer ? W.numer + o : nullptr;
+            Q.numer2 = W.numer2 ? W.numer2 + o : nullptr;
         }
         Q.beta = W.beta + f0 * J; Q.hDelta = W.hDelta + f0 * J;
         Q.var = W.var + f0; Q.initBound = W.initBound + f0; Q.held = W.held + f0;

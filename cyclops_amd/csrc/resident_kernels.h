@@ -37,6 +37,37 @@ __device__ __forceinline__ Mon monCombine(const Mon& a, const Mon& b) {
     return c;
 }
 
+// Record of a contiguous pid range for the Breslow sweep of one column
+// (ModelSpecifics.hpp:904-962 accumulates, pid by pid, D = sum of w e^xb, N1 = sum of
+// w x e^xb and N2 = sum of w x^2 e^xb from the stratum start and adds
+// nW N1 / D to the gradient and nW (N2 / D - (N1 / D)^2) to the Hessian). With N1,
+// N2 counted from the range start (D already carries the ranges before):
+//   gl = sum a N1, hl = sum (a N2 - b N1^2) with a = nW / D, b = a / D;
+//   af, bf, xf = sums of a, b, b N1 over the pids before the range's first stratum
+//   start, which a numerator carry (K1, K2) from the left reaches;
+//   t1, t2, f = numerator totals after the range's last start, and whether it has one.
+// Joining L then R adds L's numerator totals to R's first stratum:
+//   gl = L.gl + R.gl + L.t1 R.af, hl = L.hl + R.hl + L.t2 R.af - 2 L.t1 R.xf - L.t1^2 R.bf,
+// so the records of all ranges joined in order give the column's g and h.
+struct SweepRec {
+    double gl, hl, af, bf, xf, t1, t2;
+    int f, pad;
+};
+__device__ __forceinline__ SweepRec recZero() { return SweepRec{0, 0, 0, 0, 0, 0, 0, 0, 0}; }
+__device__ __forceinline__ SweepRec recCombine(const SweepRec& L, const SweepRec& R) {
+    SweepRec u;
+    u.gl = L.gl + R.gl + L.t1 * R.af;
+    u.hl = L.hl + R.hl + L.t2 * R.af - 2.0 * L.t1 * R.xf - L.t1 * L.t1 * R.bf;
+    u.af = L.f ? L.af : L.af + R.af;
+    u.bf = L.f ? L.bf : L.bf + R.bf;
+    u.xf = L.f ? L.xf : L.xf + R.xf + L.t1 * R.bf;
+    u.t1 = R.f ? R.t1 : L.t1 + R.t1;
+    u.t2 = R.f ? R.t2 : L.t2 + R.t2;
+    u.f = L.f | R.f;
+    u.pad = 0;
+    return u;
+}
+
 template <typename Real>
 struct Params {
     // data
@@ -76,6 +107,12 @@ struct Params {
     double* R1;           // [N*BP]
     double* R2;           // [N*BP]
     double* nW;           // [N*BP]
+    // Cox with one fit per team (the accumulated sweep): the column's per-pid
+    // numerators, zero outside a coordinate, and the last stratum start of each
+    // wave range of each workgroup (-1: none)
+    double* numer;        // [N]
+    double* numer2;       // [N]
+    const int* waveLast;  // [G*WAVES]
     double* beta;         // [B*J]
     double* hDelta;       // [B*J]
     const double* var;    // [B]
@@ -105,7 +142,7 @@ struct Params {
     double* ll;           // [B]
     // sync / scratch
     unsigned* epoch;      // [G]
-    double* slots;        // [2][G][BP][6]
+    double* slots;        // [2][G][BP][SLOTW]
     double* bcast;        // [2][BP][4]
     int* err;
     unsigned long long deadlineTicks;
@@ -115,7 +152,7 @@ struct Params {
     double* stats;        // [0] compulsory bytes moved; [1..5] phase ticks of workgroup 0
 };
 
-constexpr int SLOTW = 6;
+constexpr int SLOTW = 8;   // doubles per slot: a Mon (6) or a sweep record (8)
 
 __device__ __forceinline__ unsigned long long now() { return wall_clock64(); }
 
@@ -165,6 +202,21 @@ __device__ __forceinline__ bool waitAll(const unsigned* epoch, unsigned seq, int
     if (__syncthreads_or(bad)) return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return true;
+}
+
+// change d of the denominator of pid p: counted into its wave range's total when p
+// lies after the range's last stratum start
+template <typename Real>
+__device__ __forceinline__ void sweepDelta(const Params<Real>& P, int64_t p, double d, double* wsum) {
+    const int64_t q0 = P.pidStart[P.w], q1 = P.pidStart[P.w + 1];
+    const int64_t cw = (q1 - q0 + WAVES - 1) / WAVES;
+    if (cw == 0) return;
+    const int wv = (int)((p - q0) / cw);
+    if (p >= P.waveLast[P.w * WAVES + wv]) {
+#pragma unroll
+        for (int i = 0; i < WAVES; ++i)
+            if (i == wv) wsum[i] += d;
+    }
 }
 
 template <typename Real>
@@ -340,7 +392,7 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
                 if constexpr (LO) bt.xl[u][i] = go ? (Real)bf16Value(P.xbLo[(int64_t)bt.kk[u] * BP + f]) : Real(0);
             }
             if constexpr (MODEL != M_CLR) bt.ww[u][i] = go ? weightOf(P, bt.kk[u], f) : Real(0);
-            if constexpr (GROUPED) {
+            if constexpr (GROUPED && !(MODEL == M_COX && BP == 1)) {   // (the Cox sweep reads no ratios)
                 const size_t q = (size_t)bt.pp[u] * BP + f;
                 bt.r1[u][i] = go ? P.R1[q] : 0.0;
                 bt.r2[u][i] = go ? P.R2[q] : 0.0;
@@ -468,7 +520,7 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n
 // the fits that did not move with the values they had.
 template <typename Real, int MODEL, int BP, int U, bool LOW>
 __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Real, MODEL, BP, U>& bt,
-                                           const double (&dl)[GeoM<BP, MODEL>::FPL]) {
+                                           const double (&dl)[GeoM<BP, MODEL>::FPL], double* wsum = nullptr) {
     using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     constexpr bool COMP = sizeof(Real) == 4;   // compensated xb
@@ -535,6 +587,8 @@ __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Re
                     const size_t q = (size_t)p * BP + f;
                     const double dn = P.denom[q] + acc[i];
                     P.denom[q] = dn;
+                    if constexpr (MODEL == M_COX && BP == 1)
+                        if (wsum) sweepDelta(P, p, acc[i], wsum);
                     if constexpr (MODEL == M_CLR) {
                         const double nw = P.nW[q];
                         const double r = nw == 0.0 ? 0.0 : nw / dn;
@@ -550,7 +604,7 @@ __device__ __forceinline__ void applyBatch(const Params<Real>& P, const Batch<Re
 
 template <typename Real, int MODEL, int BP, int FMT>
 __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t n0, int64_t n1,
-                                           const double (&dl)[GeoM<BP, MODEL>::FPL]) {
+                                           const double (&dl)[GeoM<BP, MODEL>::FPL], double* wsum = nullptr) {
     using GE = GeoM<BP, MODEL>;
     constexpr int U = UnrollM<BP, MODEL>::U;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
@@ -654,6 +708,8 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
                         const size_t q = (size_t)p * BP + f;
                         const double dn = P.denom[q] + acc[i];
                         P.denom[q] = dn;
+                        if constexpr (MODEL == M_COX && BP == 1)
+                            if (wsum) sweepDelta(P, p, acc[i], wsum);
                         if constexpr (MODEL == M_CLR) {
                             const double nw = P.nW[q];
                             const double r = nw == 0.0 ? 0.0 : nw / dn;
@@ -752,6 +808,11 @@ __device__ __forceinline__ Mon shflDown(const Mon& v, int d) {
     return Mon{__shfl_down(v.g, d, 64), __shfl_down(v.h, d, 64), __shfl_down(v.l, d, 64),
                __shfl_down(v.m, d, 64), __shfl_down(v.t, d, 64), __shfl_down(v.r, d, 64)};
 }
+__device__ __forceinline__ SweepRec shflDown(const SweepRec& v, int d) {
+    return SweepRec{__shfl_down(v.gl, d, 64), __shfl_down(v.hl, d, 64), __shfl_down(v.af, d, 64),
+                    __shfl_down(v.bf, d, 64), __shfl_down(v.xf, d, 64), __shfl_down(v.t1, d, 64),
+                    __shfl_down(v.t2, d, 64), __shfl_down(v.f, d, 64), 0};
+}
 __device__ __forceinline__ Seg2 shflDown(const Seg2& v, int d) {
     Seg2 r;
     r.a = __shfl_down(v.a, d, 64);
@@ -797,7 +858,15 @@ struct Shared {
     union {
         Mon red[MAXG];
         Seg2 seg[MAXG];
+        SweepRec rec[WAVES + 1];
+        double part[BLOCK * WAVES];
     } u;
+    // Cox sweep (one fit per team): segmented denominator total of each wave range
+    // (kept exact by the walks, moved by the updates in between) and the walk's
+    // per-wave records
+    Seg2 waveDen[WAVES];
+    Seg2 waveDenNew[WAVES];
+    SweepRec wrec[WAVES];
     Seg2 scan[Geo<BP>::NSG * BP];
     double out[BP * 4];
     double beta[BP];
@@ -1103,7 +1172,10 @@ __device__ __forceinline__ bool coxRescanWaves(const Params<Real>& P, unsigned& 
     // which also forms the backward totals)
     Seg2 none = z;
     Seg2 mine = fwdWalk(z, false, none);
-    if (lane == 0) s.scan[wave] = mine;
+    if (lane == 0) {
+        s.scan[wave] = mine;
+        s.waveDen[wave] = mine;   // the sweep's exact wave totals
+    }
     __syncthreads();
     Seg2 carry;
     if (!exchange(false, carry)) return false;
@@ -1449,6 +1521,269 @@ __device__ __forceinline__ bool coxRescanAny(const Params<Real>& P, unsigned& se
     else return coxRescan<Real, BP>(P, seq, deadline, s);
 }
 
+// ---------------------------------------------------------------------------
+// Breslow Cox, one fit per team: the accumulated sweep. Per coordinate the
+// column's numerators w x e^xb (and w x^2 e^xb) are scattered to their pids, one
+// forward walk over every pid of the team forms D, N1, N2 and the partial
+// gradient records, and one exchange joins the records (with a second exchange for
+// the denominator carries only after an update moved the denominators). This is
+// the pid-by-pid accumulation of ModelSpecifics.hpp:904-962 (computeGradientAndHessian
+// for the accumulated models) spread over the workgroups: nothing per pid is written
+// per coordinate, where the telescoped form rewrites the risk-set ratios of every pid.
+// ---------------------------------------------------------------------------
+
+// wave range [a, b) of wave `wave` of workgroup w (the split the rescan uses)
+template <typename Real>
+__device__ __forceinline__ void waveRange(const Params<Real>& P, int wave, int64_t& a, int64_t& b) {
+    const int64_t q0 = P.pidStart[P.w], q1 = P.pidStart[P.w + 1];
+    const int64_t cw = (q1 - q0 + WAVES - 1) / WAVES;
+    a = q0 + wave * cw;
+    b = a + cw;
+    if (a > q1) a = q1;
+    if (b > q1) b = q1;
+}
+
+// Adds every thread's denominator changes to the wave totals in a fixed order.
+template <int BP>
+__device__ __forceinline__ void sweepAddWaveDeltas(Shared<BP>& s, const double (&wsum)[WAVES]) {
+#pragma unroll
+    for (int i = 0; i < WAVES; ++i) s.u.part[threadIdx.x * WAVES + i] = wsum[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < BLOCK / 64; ++q) v += s.u.part[(q * 64 + lane) * WAVES + wv];
+    v = waveSumDpp(v);
+    if (lane == 0) s.waveDen[wv].a += v;
+    __syncthreads();
+}
+
+// Numerators of the entries of a register batch, summed per run of equal pid and
+// stored at the pid (clear: store zeros at the same pids).
+template <typename Real, int MODEL, int BP, int U>
+__device__ __forceinline__ void sweepScatterBatch(const Params<Real>& P, const Batch<Real, MODEL, BP, U>& bt,
+                                                  bool two, bool clear) {
+    double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!bt.val[u]) break;
+        if (!clear) {
+            const Real x = bt.xx[u];
+            const Real c = bt.ww[u][0] * dexp(bt.xv[u][0]) * x;
+            c1 += (double)c;
+            c2 += (double)(c * x);
+        }
+        const int pn = (u + 1 < U && bt.val[u + 1]) ? bt.pp[u + 1] : -1;
+        if (pn != bt.pp[u]) {   // last entry of a run of equal pid
+            P.numer[bt.pp[u]] = c1;
+            if (two) P.numer2[bt.pp[u]] = c2;
+            c1 = c2 = 0.0;
+        }
+    }
+}
+
+// The same over a chunk [n0, n1) of any length (runs may cross batches).
+template <typename Real, int MODEL, int BP, int FMT, int U>
+__device__ __forceinline__ void sweepScatterPass(const Params<Real>& P, int j, int64_t n0, int64_t n1,
+                                                 const bool (&on)[1], bool two, bool clear) {
+    double c1 = 0.0, c2 = 0.0;
+    for (int64_t e = n0; e < n1; e += U) {
+        Batch<Real, MODEL, BP, U> bt;
+        loadRows<Real, MODEL, BP, FMT, U>(P, j, e, n1, bt);
+        if (!clear) loadState<Real, MODEL, BP, U>(P, on, bt);
+        const int pAfter = (e + U < n1) ? P.pid[rowAt<Real, FMT>(P, e + U)] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!bt.val[u]) break;
+            if (!clear) {
+                const Real x = bt.xx[u];
+                const Real c = bt.ww[u][0] * dexp(bt.xv[u][0]) * x;
+                c1 += (double)c;
+                c2 += (double)(c * x);
+            }
+            const int pn = (u + 1 < U) ? (bt.val[u + 1] ? bt.pp[u + 1] : -1) : pAfter;
+            if (pn != bt.pp[u]) {
+                P.numer[bt.pp[u]] = c1;
+                if (two) P.numer2[bt.pp[u]] = c2;
+                c1 = c2 = 0.0;
+            }
+        }
+    }
+}
+
+// Carry of the denominators into this workgroup: its wave totals joined and
+// published (right after an update, so the exchange overlaps the next column's
+// scatter), then, once every workgroup has published, the segmented prefix of
+// the workgroups before this one.
+template <typename Real, int BP>
+__device__ __forceinline__ void sweepCarryPublish(const Params<Real>& P, unsigned& seq, Shared<BP>& s) {
+    if (threadIdx.x == 0) {
+        Seg2 tot{0.0, 0.0, 0, 0};
+        for (int i = 0; i < WAVES; ++i) tot = segCombine(tot, s.waveDen[i]);
+        double* o = P.slots + ((size_t)((seq + 1) & 1) * P.G + P.w) * SLOTW;
+        ast(o, tot.a);
+        ast(o + 1, (double)tot.f);
+    }
+    publish(P.epoch, ++seq, P.w);
+}
+template <typename Real, int BP>
+__device__ __forceinline__ bool sweepCarryWait(const Params<Real>& P, unsigned seq, unsigned long long deadline,
+                                               Shared<BP>& s, double& carryD) {
+    const Seg2 z{0.0, 0.0, 0, 0};
+    if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+    const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * SLOTW;
+    const Seg2 c = treeReduce<Seg2>(P.w, [&](int q) {
+        const double* o = sl2 + (size_t)q * SLOTW;
+        return Seg2{ald(o), 0.0, (int)ald(o + 1), 0};
+    }, [](const Seg2& x, const Seg2& y) { return segCombine(x, y); }, s.u.seg, z);
+    carryD = c.a;
+    return true;
+}
+
+// The walk of one wave over its pid range: D from the carry, N1 and N2 from the
+// range start; leaves the wave's record and its exact denominator total in LDS.
+// Chunks of CH consecutive pids per lane, one chunk of loads in flight.
+template <typename Real, int BP>
+__device__ __forceinline__ void sweepWalk(const Params<Real>& P, Shared<BP>& s, double carryD, bool two) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t a, b;
+    waveRange(P, wave, a, b);
+    const int64_t N = P.N;
+    const unsigned char* fl = P.pflag;
+    constexpr int CH = 8;
+    constexpr int CW = 64 * CH;
+    Seg2 dIn{carryD, 0.0, 0, 0};
+    for (int i = 0; i < wave; ++i) dIn = segCombine(dIn, s.waveDen[i]);
+    const Seg3 z3{0.0, 0.0, 0.0, 0};
+    Seg3 run{dIn.a, 0.0, 0.0, 0};   // flag and numerators count from the range start
+    Seg2 own{0.0, 0.0, 0, 0};
+    double gl = 0.0, hl = 0.0, af = 0.0, bf = 0.0, xf = 0.0;
+    // Loads are coalesced (element e of a chunk, pid base + e, by lane e % 64) and
+    // each chunk is turned around in this wave's LDS so that a lane then holds CH
+    // consecutive pids: 64 consecutive doubles per load instruction instead of one
+    // 64-byte block per lane, whose 64 separate cache lines per instruction held the
+    // walk at the rate the memory pipeline accepts requests.
+    constexpr int SP = CW + CW / CH;   // element e at e + e / CH: strided reads spread over the banks
+    __shared__ double sD[WAVES][SP], sW[WAVES][SP], sN[WAVES][SP], sM[WAVES][SP];
+    __shared__ unsigned char sF[WAVES][CW];
+    auto px = [](int e) { return e + e / CH; };
+    struct Raw {
+        double d[CH], n[CH], w[CH], m[CH];
+        int f[CH];
+    };
+    auto fetch = [&](int64_t base, Raw& r) {
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int64_t q = base + i * 64 + lane;
+            const int64_t p = q < b ? q : b - 1;
+            r.d[i] = P.denom[p];
+            r.w[i] = P.nW[p];
+            r.n[i] = P.numer[p];
+            if (two) r.m[i] = P.numer2[p];
+            r.f[i] = fl[p];
+        }
+    };
+    auto waveSync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    Raw nxt;
+    if (a < b) fetch(a, nxt);
+    for (int64_t base = a; base < b; base += CW) {
+        Raw cur;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {   // into LDS in load order
+            const int e = i * 64 + lane;
+            sD[wave][px(e)] = nxt.d[i];
+            sW[wave][px(e)] = nxt.w[i];
+            sN[wave][px(e)] = nxt.n[i];
+            if (two) sM[wave][px(e)] = nxt.m[i];
+            sF[wave][e] = (unsigned char)nxt.f[i];
+        }
+        if (base + CW < b) fetch(base + CW, nxt);
+        waveSync();
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {   // out in lane order: this lane's CH consecutive pids
+            const int e = lane * CH + i;
+            cur.d[i] = sD[wave][px(e)];
+            cur.w[i] = sW[wave][px(e)];
+            cur.n[i] = sN[wave][px(e)];
+            if (two) cur.m[i] = sM[wave][px(e)];
+            cur.f[i] = sF[wave][e];
+        }
+        waveSync();   // read before the next chunk is written
+        const int64_t p0 = base + (int64_t)lane * CH;
+        auto elem = [&](int i) {
+            const int64_t p = p0 + i;
+            const bool ok = p < b;
+            const bool st = ok && p > 0 && cur.f[i] != 0;
+            return Seg3{ok ? cur.d[i] : 0.0, ok ? cur.n[i] : 0.0, ok ? (two ? cur.m[i] : cur.n[i]) : 0.0, st ? 1 : 0};
+        };
+        Seg3 tot = z3;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) tot = seg3Combine(tot, elem(i));
+        const Seg3 inc = waveSeg3ScanDpp(tot);
+        const Seg3 excl = seg3FromPrevLane(inc);
+        const Seg3 all = seg3Lane63(inc);
+        Seg3 acc = seg3Combine(run, excl);
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            acc = seg3Combine(acc, elem(i));
+            const double nw = cur.w[i];
+            if (p0 + i < b && nw != 0.0) {
+                const double inv = 1.0 / acc.d;
+                const double ra = nw * inv, rb = ra * inv;
+                gl += ra * acc.n1;
+                hl += ra * acc.n2 - rb * acc.n1 * acc.n1;
+                if (!acc.f) {   // before the range's first stratum start
+                    af += ra;
+                    bf += rb;
+                    xf += rb * acc.n1;
+                }
+            }
+        }
+        run = seg3Combine(run, all);
+        own = segCombine(own, Seg2{all.d, 0.0, all.f, 0});
+    }
+    gl = waveSumDpp(gl);
+    hl = waveSumDpp(hl);
+    af = waveSumDpp(af);
+    bf = waveSumDpp(bf);
+    xf = waveSumDpp(xf);
+    if (lane == 0) {
+        s.wrec[wave] = SweepRec{gl, hl, af, bf, xf, run.n1, run.n2, run.f, 0};
+        s.waveDenNew[wave] = own;
+    }
+    __syncthreads();
+}
+
+// Joins the wave records into the workgroup's, exchanges, and joins every
+// workgroup's in order: the column's accumulated g and h (in every workgroup).
+template <typename Real, int BP>
+__device__ __forceinline__ bool sweepReduce(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
+                                            Shared<BP>& s, double& g, double& h) {
+    if (threadIdx.x == 0) {
+        SweepRec r = s.wrec[0];
+        for (int i = 1; i < WAVES; ++i) r = recCombine(r, s.wrec[i]);
+        double* o = P.slots + ((size_t)((seq + 1) & 1) * P.G + P.w) * SLOTW;
+        ast(o, r.gl); ast(o + 1, r.hl); ast(o + 2, r.af); ast(o + 3, r.bf);
+        ast(o + 4, r.xf); ast(o + 5, r.t1); ast(o + 6, r.t2); ast(o + 7, (double)r.f);
+    }
+    if (threadIdx.x < WAVES) s.waveDen[threadIdx.x] = s.waveDenNew[threadIdx.x];
+    publish(P.epoch, ++seq, P.w);
+    if (!waitAll(P.epoch, seq, P.G, P.err, deadline)) return false;
+    const double* sl2 = P.slots + (size_t)(seq & 1) * P.G * SLOTW;
+    const SweepRec t = treeReduce<SweepRec>(P.G, [&](int q) {
+        const double* o = sl2 + (size_t)q * SLOTW;
+        return SweepRec{ald(o), ald(o + 1), ald(o + 2), ald(o + 3), ald(o + 4), ald(o + 5), ald(o + 6),
+                        (int)ald(o + 7), 0};
+    }, [](const SweepRec& x, const SweepRec& y) { return recCombine(x, y); }, s.u.rec, recZero());
+    g = t.gl;
+    h = t.hl;
+    return true;
+}
+
 // The persistent findMode kernel (CyclicCoordinateDescent.cpp:909-1216 for every fit).
 // Launch = teams of T workgroups; team t runs the fits described by teams[t]
 // (its own data and state pointers), exchanging only inside the team.
@@ -1459,6 +1794,9 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     // from the scalar cache where used instead of being held in registers
     const Params<Real>& P = teams[blockIdx.x];
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
+    // Cox with one fit per team runs the accumulated sweep (every workgroup computes
+    // the step: one fit per team is always all-read)
+    constexpr bool SWEEP = MODEL == M_COX && BP == 1;
     __shared__ Shared<BP> s;
     const int w = P.w;
     const unsigned long long deadline = P.deadlineTicks;   // per-wait limit
@@ -1486,6 +1824,8 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 // the pass above wrote only this workgroup's pids; the scan reads them all
                 if (!coxRescanAny<Real, BP>(P, seq, deadline, s)) return;
             }
+        } else if constexpr (SWEEP) {
+            if (!coxRescanAny<Real, BP>(P, seq, deadline, s)) return;   // the sweep's wave totals (LDS)
         }
     }
     const int objOp = P.convType == CCD_CONV_GRADIENT ? OBJ_GRADIENT
@@ -1539,6 +1879,9 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     // compensated xb: the low parts travel with the registers for independent-row
     // models; grouped models, whose update also walks per-pid state, fetch them there
     constexpr bool LOW = sizeof(Real) == 4 && !GROUPED;
+    double carryD = 0.0;        // sweep: denominator carry into this workgroup
+    bool denomStale = true;     // sweep: the wave totals were recomputed; no carry published
+    bool carryPending = false;  // sweep: this workgroup's totals are published, not yet joined
     auto fitsBatch = [&](const Meta& m) {   // every sub-group chunk of m has <= UF entries
         return (m.b - m.a + GE::NSG - 1) / GE::NSG <= UF;
     };
@@ -1618,6 +1961,99 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     if (mine(f)) step(f, monZero(), s.out + 4 * f);
                 prefetchRows();
                 __syncthreads();
+            } else if constexpr (SWEEP) {
+                const unsigned long long tA = now();
+                int64_t n0, n1;   // this thread's run-aligned chunk of the column
+                switch (fm) {
+                case F_INDICATOR: subChunk<Real, F_INDICATOR, true>(P, cur.a, cur.b, threadIdx.x, BLOCK, n0, n1); break;
+                case F_SPARSE: subChunk<Real, F_SPARSE, true>(P, cur.a, cur.b, threadIdx.x, BLOCK, n0, n1); break;
+                case F_DENSE: subChunk<Real, F_DENSE, true>(P, cur.a, cur.b, threadIdx.x, BLOCK, n0, n1); break;
+                default: subChunk<Real, F_INTERCEPT, true>(P, cur.a, cur.b, threadIdx.x, BLOCK, n0, n1); break;
+                }
+                const bool on[1] = {P.B > 0 && s.active[0] != 0};
+                const bool two = fm == F_SPARSE || fm == F_DENSE;   // x^2 != x
+                // a chunk of at most UF entries stays in registers until the update
+                const bool fast = __syncthreads_and(n1 - n0 <= UF);
+                Batch<Real, MODEL, BP, UF> bt;
+                auto scatter = [&](bool clear) {
+                    if (fast) {
+                        sweepScatterBatch<Real, MODEL, BP, UF>(P, bt, two, clear);
+                    } else {
+                        switch (fm) {
+                        case F_INDICATOR: sweepScatterPass<Real, MODEL, BP, F_INDICATOR, UF>(P, j, n0, n1, on, two, clear); break;
+                        case F_SPARSE: sweepScatterPass<Real, MODEL, BP, F_SPARSE, UF>(P, j, n0, n1, on, two, clear); break;
+                        case F_DENSE: sweepScatterPass<Real, MODEL, BP, F_DENSE, UF>(P, j, n0, n1, on, two, clear); break;
+                        default: sweepScatterPass<Real, MODEL, BP, F_INTERCEPT, UF>(P, j, n0, n1, on, two, clear); break;
+                        }
+                    }
+                };
+                if (fast) {
+                    switch (fm) {
+                    case F_INDICATOR: loadBatch<Real, MODEL, BP, F_INDICATOR, UF>(P, j, n0, n1, on, bt); break;
+                    case F_SPARSE: loadBatch<Real, MODEL, BP, F_SPARSE, UF>(P, j, n0, n1, on, bt); break;
+                    case F_DENSE: loadBatch<Real, MODEL, BP, F_DENSE, UF>(P, j, n0, n1, on, bt); break;
+                    default: loadBatch<Real, MODEL, BP, F_INTERCEPT, UF>(P, j, n0, n1, on, bt); break;
+                    }
+                }
+                scatter(false);
+                __syncthreads();   // the numerators are stored before any wave walks
+                if (denomStale) {
+                    sweepCarryPublish<Real, BP>(P, seq, s);
+                    carryPending = true;
+                    denomStale = false;
+                }
+                if (carryPending) {
+                    if (!sweepCarryWait<Real, BP>(P, seq, deadline, s, carryD)) return;
+                    carryPending = false;
+                }
+                sweepWalk<Real, BP>(P, s, carryD, two);
+                const unsigned long long tB = now();
+
+                double g = 0.0, h = 0.0;
+                if (!sweepReduce<Real, BP>(P, seq, deadline, s, g, h)) return;
+                if (threadIdx.x == 0) step(0, Mon{g, h, 0.0, 0.0, 0.0, 0.0}, s.out);
+                __syncthreads();
+                const unsigned long long tC = now();
+                scatter(true);   // this workgroup's walk is over: numerators back to zero
+                double dl[1] = {s.out[0]};
+                if (dl[0] != 0.0) {
+                    double wsum[WAVES];
+#pragma unroll
+                    for (int i = 0; i < WAVES; ++i) wsum[i] = 0.0;
+                    if (fast) {
+                        applyBatch<Real, MODEL, BP, UF, false>(P, bt, dl, wsum);
+                    } else {
+                        switch (fm) {
+                        case F_INDICATOR: updatePass<Real, MODEL, BP, F_INDICATOR>(P, j, n0, n1, dl, wsum); break;
+                        case F_SPARSE: updatePass<Real, MODEL, BP, F_SPARSE>(P, j, n0, n1, dl, wsum); break;
+                        case F_DENSE: updatePass<Real, MODEL, BP, F_DENSE>(P, j, n0, n1, dl, wsum); break;
+                        default: updatePass<Real, MODEL, BP, F_INTERCEPT>(P, j, n0, n1, dl, wsum); break;
+                        }
+                    }
+                    __syncthreads();
+                    sweepAddWaveDeltas<BP>(s, wsum);
+                    sweepCarryPublish<Real, BP>(P, seq, s);
+                    carryPending = true;
+                }
+                if (w == 0 && threadIdx.x == 0) {
+                    const unsigned long long tD = now();
+                    loc[1] += (double)(tB - tA);
+                    loc[2] += (double)(tC - tB);
+                    loc[3] += (double)(tD - tC);
+                    locSteps += 1;
+                    // compulsory bytes: the column's ids / values / weights, xb and pid
+                    // of its rows and the numerator stores; the walk reads D, nW and
+                    // the numerators of every pid; an update reads and writes xb (and
+                    // its low part) and the denominators of its pids
+                    const double n = (double)nnzj;
+                    const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
+                                             (two ? sizeof(Real) : 0.0) + ((P.rowFold || P.kw) ? 4.0 : 0.0));
+                    constexpr double lowB = sizeof(Real) == 4 ? 2.0 : 0.0;
+                    double by = base + n * (sizeof(Real) + 4.0 + (two ? 16.0 : 8.0)) +
+                                (double)P.N * (two ? 33.0 : 25.0);
+                    if (dl[0] != 0.0) by += base + n * (2.0 * (sizeof(Real) + lowB) + 4.0 + 16.0);
+                    loc[0] += by;
+                }
             } else {
                 const unsigned long long tA = now();
                 const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
@@ -1743,6 +2179,14 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
         }
         // convergence test of every fit (performCheckConvergence :839-906)
         const unsigned long long tO = now();
+        if constexpr (SWEEP) {   // D of every pid for the objective (and exact wave totals)
+            if (carryPending) {   // the last update's exchange, joined so the sequence stays in step
+                if (!sweepCarryWait<Real, BP>(P, seq, deadline, s, carryD)) return;
+                carryPending = false;
+            }
+            if (!coxRescanAny<Real, BP>(P, seq, deadline, s)) return;
+            denomStale = true;
+        }
         objectivePartial<Real, MODEL, BP>(P, objOp, s);
         writeSlots<Real, BP>(P, seq, s, false);
         if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
