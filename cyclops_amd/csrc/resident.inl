@@ -676,10 +676,14 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     P.priorType = a.prior_type; P.convType = a.convergence_type; P.tol = a.tolerance; P.maxIt = a.max_iterations;
     P.err = W.err; P.allDoneOut = W.allDone; P.steps = W.steps; P.stats = W.stats;
     P.deadlineTicks = (unsigned long long)rateKHz * 1000ull * 20ull;   // 20 s for one grid exchange
+    P.timing = std::getenv("CCD_PHASES") ? 1 : 0;
     // all-read: every workgroup reduces and steps every fit after one exchange; used
     // when the partials to read per coordinate are few (G x fits), else the owner of
     // each fit steps it and broadcasts the delta (a second exchange)
     P.allRead = (L.Bt <= 2 || G == 1 || (!grouped && (int64_t)G * BP <= 1024)) ? 1 : 0;
+    // FP32 runs of independent rows in all-read teams: tagged per-coordinate partials
+    // (no epoch exchange per coordinate); the slot halves are reset before every launch
+    P.tagged = (sizeof(Real) == 4 && !grouped && P.allRead && G > 1 && (int64_t)G * BP <= 1024) ? 1 : 0;
 
     std::vector<res::Params<Real>> teamP(nTeams, P);
     for (int t = 0; t < nTeams; ++t) {
@@ -778,8 +782,19 @@ struct Launcher {
         auto& W0 = *R0.W;
         hipStream_t s = R0.E->s;
         const int one = 1;
-        for (auto* R : runs)
+        for (auto* R : runs) {
             CCD_HIP(hipMemsetAsync(R->W->epoch, 0, (size_t)R->nTeams * R->G * sizeof(unsigned), s));
+            if (R->teamP[0].tagged) {
+                // tagged slot words: half 0 with tag bit 0, half 1 with tag bit 1, so that
+                // neither matches the tag of its first exchange in the launch
+                const size_t half = (size_t)R->G * R->BP * res::SLOTW;
+                for (int t = 0; t < R->nTeams; ++t) {
+                    double* base = R->W->slots + (size_t)t * 2 * half;
+                    CCD_HIP(hipMemsetAsync(base, 0x00, half * sizeof(double), s));
+                    CCD_HIP(hipMemsetAsync(base + half, 0xFF, half * sizeof(double), s));
+                }
+            }
+        }
         CCD_HIP(hipMemsetAsync(W0.err, 0, sizeof(int), s));
         h2d(W0.allDone, &one, 1, s);
         if (launches <= 1 || budget != lastBudget) {   // the first launch starts the fits, later ones resume

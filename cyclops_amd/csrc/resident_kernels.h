@@ -147,6 +147,8 @@ struct Params {
     int* err;
     unsigned long long deadlineTicks;
     int allRead;
+    int timing;           // phase timers of workgroup 0 (stats[1..4]) are recorded
+    int tagged;           // per-coordinate partials go through tagged slot words (writeTagged)
     // counters
     unsigned long long* steps;
     double* stats;        // [0] compulsory bytes moved; [1..5] phase ticks of workgroup 0
@@ -617,17 +619,44 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
         acc[i] = 0.0;
         anyLane = anyLane || dl[i] != 0.0;
     }
+    // independent rows: the next step's row ids and values are requested before this
+    // step's gathers, so the two latencies overlap (one round trip per step)
+    int64_t nk[U];
+    Real nx[U];
+    bool nv[U];
+    if constexpr (!GROUPED) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            nv[u] = n0 + u < n1;
+            nk[u] = nv[u] ? rowAt<Real, FMT>(P, n0 + u) : 0;
+            nx[u] = nv[u] ? xAt<Real, FMT>(P, j, n0 + u) : Real(0);
+        }
+    }
     for (int64_t e = n0; e < n1; e += U) {
         int64_t kk[U];
         Real xx[U], xv[U][GE::FPL], xl[U][GE::FPL], ww[U][GE::FPL];
         int pp[U], pn[U];
         bool val[U];
+        if constexpr (!GROUPED) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            val[u] = e + u < n1;
-            kk[u] = val[u] ? rowAt<Real, FMT>(P, e + u) : 0;
-            xx[u] = val[u] ? xAt<Real, FMT>(P, j, e + u) : Real(0);
-            if constexpr (GROUPED) {
+            for (int u = 0; u < U; ++u) {
+                kk[u] = nk[u];
+                xx[u] = nx[u];
+                val[u] = nv[u];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t q = e + U + u;
+                nv[u] = q < n1;
+                nk[u] = nv[u] ? rowAt<Real, FMT>(P, q) : 0;
+                nx[u] = nv[u] ? xAt<Real, FMT>(P, j, q) : Real(0);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                val[u] = e + u < n1;
+                kk[u] = val[u] ? rowAt<Real, FMT>(P, e + u) : 0;
+                xx[u] = val[u] ? xAt<Real, FMT>(P, j, e + u) : Real(0);
                 pp[u] = val[u] ? P.pid[kk[u]] : -1;
                 pn[u] = (val[u] && e + u + 1 < n1) ? P.pid[rowAt<Real, FMT>(P, e + u + 1)] : -1;
             }
@@ -886,7 +915,7 @@ struct Shared {
 // s.out[4f..4f+3] holds fn's output for every fit in every workgroup.
 template <typename Real, int BP, class Fn>
 __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq, unsigned long long deadline,
-                                           Shared<BP>& s, Fn fn, bool ordered = true) {
+                                           Shared<BP>& s, Fn fn, bool ordered = true, int off = 0) {
     if (P.G == 1) {   // a one-workgroup team reduces in LDS only; fits step in parallel
         for (int f = threadIdx.x; f < P.B; f += BLOCK) fn(f, s.mon[f], s.out + 4 * f);
         __syncthreads();
@@ -904,7 +933,7 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
         double g = 0.0, h = 0.0;
         if (f < P.B)
             for (int w = part; w < P.G; w += TPF) {
-                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+                const double* q = slot + ((size_t)w * BP + f) * SLOTW + off;
                 g += ald(q);
                 h += ald(q + 1);
             }
@@ -940,7 +969,7 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
         if (!ordered) {
             double v[2] = {0.0, 0.0};
             for (int w = threadIdx.x; w < P.G; w += BLOCK) {
-                const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+                const double* q = slot + ((size_t)w * BP + f) * SLOTW + off;
                 v[0] += ald(q);
                 v[1] += ald(q + 1);
             }
@@ -979,16 +1008,142 @@ __device__ __forceinline__ bool reduceFits(const Params<Real>& P, unsigned& seq,
 // Write this workgroup's per-fit Mons (s.mon[f]) to the slots of the next epoch.
 template <typename Real, int BP>
 __device__ __forceinline__ void writeSlots(const Params<Real>& P, unsigned seq, const Shared<BP>& s,
-                                           bool ordered = true) {
+                                           bool ordered = true, int off = 0) {
     if (P.G == 1) return;
     double* slot = P.slots + (size_t)((seq + 1) & 1) * P.G * BP * SLOTW;
     for (int f = threadIdx.x; f < BP; f += BLOCK) {
-        double* q = slot + ((size_t)P.w * BP + f) * SLOTW;
+        double* q = slot + ((size_t)P.w * BP + f) * SLOTW + off;
         const Mon& m = s.mon[f];
         ast(q, m.g);
         ast(q + 1, m.h);
         if (ordered) { ast(q + 2, m.l); ast(q + 3, m.m); ast(q + 4, m.t); ast(q + 5, m.r); }
     }
+}
+
+// Tagged exchange of the per-coordinate (g, h) partials: FP32 runs of independent
+// rows in all-read teams with G x BP <= 1024. Each partial is one 64-bit word whose
+// lowest mantissa bit carries the exchange's tag, (t >> 1) & 1 for exchange t, which
+// uses slot half t & 1. The stale word a reader may find in that half was written two
+// exchanges earlier, with the other tag, so a reader that sees the tag sees the value:
+// there is no epoch word, no wait for this workgroup's stores to be acknowledged, and
+// the next column's row-id loads stay in flight through the exchange. A half is
+// rewritten only after every workgroup has read it: a workgroup writes exchange t + 1
+// after it has seen every exchange-t word, each of which was written after its writer
+// had summed exchange t - 1. Before every launch the host fills half 0 with all-zero
+// bytes and half 1 with all-one bytes, so neither matches its first tag. Readers clear
+// the tag bit before summing. A partial of FP32 terms is an exact FP64 sum whose lowest
+// mantissa bit is zero unless its terms span more than 2^29, so the value read is the
+// value written and results stay bit-identical to the epoch exchange; otherwise it is
+// at most one unit in the last place of a double lower. An infinite partial (tag 1
+// makes it a NaN pattern) reads back as the same infinity. The once-per-cycle
+// objective keeps the epoch exchange on words 2-3.
+__device__ __forceinline__ double withTag(double v, unsigned long long tag) {
+    return __longlong_as_double((long long)(((unsigned long long)__double_as_longlong(v) & ~1ull) | tag));
+}
+__device__ __forceinline__ unsigned long long tagOf(double v) {
+    return (unsigned long long)__double_as_longlong(v) & 1ull;
+}
+__device__ __forceinline__ double untag(double v) { return withTag(v, 0ull); }
+
+template <typename Real, int BP>
+__device__ __forceinline__ void writeTagged(const Params<Real>& P, unsigned tseq, const Shared<BP>& s) {
+    const unsigned t = tseq + 1;
+    const unsigned long long tag = (t >> 1) & 1u;
+    double* slot = P.slots + (size_t)(t & 1u) * P.G * BP * SLOTW;
+    for (int f = threadIdx.x; f < BP; f += BLOCK) {
+        double* q = slot + ((size_t)P.w * BP + f) * SLOTW;
+        ast(q, withTag(s.mon[f].g, tag));
+        ast(q + 1, withTag(s.mon[f].h, tag));
+    }
+}
+
+// Every workgroup sums every fit (the all-read order of reduceFits: thread t reads the
+// partials w = t / BP, t / BP + BLOCK / BP, ... of fit t % BP, then lanes of one fit by
+// a butterfly and the waves in order) once all of its words carry the tag.
+template <typename Real, int BP, class Fn>
+__device__ __forceinline__ bool reduceTagged(const Params<Real>& P, unsigned& tseq, unsigned long long ticks,
+                                             Shared<BP>& s, Fn fn) {
+    ++tseq;
+    const unsigned long long tag = (tseq >> 1) & 1u;
+    const double* slot = P.slots + (size_t)(tseq & 1u) * P.G * BP * SLOTW;
+    constexpr int TPF = BLOCK / BP;
+    constexpr int MW = 1024 / BLOCK;   // G x BP <= 1024: at most 4 workgroups' words per thread
+    const int f = threadIdx.x % BP, part = threadIdx.x / BP, lane = threadIdx.x % 64;
+    double a[MW], b[MW];
+    bool wait[MW];
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+        const int w = part + i * TPF;
+        wait[i] = f < P.B && w < P.G;
+        a[i] = 0.0;
+        b[i] = 0.0;
+        if (wait[i]) {
+            const double* q = slot + ((size_t)w * BP + f) * SLOTW;
+            a[i] = ald(q);
+            b[i] = ald(q + 1);
+        }
+    }
+    int bad = 0;
+    unsigned spins = 0;
+    unsigned long long deadline = 0;
+    while (true) {
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+            if (wait[i]) {
+                if (tagOf(a[i]) == tag && tagOf(b[i]) == tag) wait[i] = false;
+                else ok = false;
+            }
+        if (ok) break;
+        if ((++spins & 63u) == 0) {   // another workgroup's timeout ends this wait too
+            if (__hip_atomic_load(P.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { bad = 1; break; }
+            if (deadline == 0) deadline = now() + ticks;
+            else if (now() > deadline) {
+                __hip_atomic_store(P.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bad = 1;
+                break;
+            }
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int i = 0; i < MW; ++i)
+            if (wait[i]) {
+                const double* q = slot + ((size_t)(part + i * TPF) * BP + f) * SLOTW;
+                a[i] = ald(q);
+                b[i] = ald(q + 1);
+            }
+    }
+    if (__syncthreads_or(bad)) return false;
+    double g = 0.0, h = 0.0;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) {
+        g += untag(a[i]);
+        h += untag(b[i]);
+    }
+    int rowsOut;
+    if constexpr (BP < 64) {
+#pragma unroll
+        for (int o = BP; o < 64; o <<= 1) {
+            g += __shfl_xor(g, o, 64);
+            h += __shfl_xor(h, o, 64);
+        }
+        if (lane < BP) s.u.red[(threadIdx.x / 64) * BP + f] = Mon{g, h, 0, 0, 0, 0};
+        rowsOut = BLOCK / 64;
+    } else {
+        s.u.red[part * BP + f] = Mon{g, h, 0, 0, 0, 0};
+        rowsOut = TPF;
+    }
+    __syncthreads();
+    for (int ff = threadIdx.x; ff < P.B; ff += BLOCK) {
+        Mon m = s.u.red[ff];
+        for (int q = 1; q < rowsOut; ++q) {
+            m.g += s.u.red[q * BP + ff].g;
+            m.h += s.u.red[q * BP + ff].h;
+        }
+        fn(ff, m, s.out + 4 * ff);
+    }
+    __syncthreads();
+    return true;
 }
 
 // Breslow risk sets of this workgroup's pids, both scans, with the cross-workgroup carries.
@@ -1802,6 +1957,12 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     const unsigned long long deadline = P.deadlineTicks;   // per-wait limit
     unsigned seq = 0;
     const bool own = P.allRead != 0;   // control state in LDS (all-read) or global (owner mode)
+    // FP32 runs of independent rows: per-coordinate partials through tagged slot words
+    // (writeTagged); the objective's epoch exchanges then use words 2-3 of each slot
+    constexpr bool TAGGABLE = !GROUPED && sizeof(Real) == 4;
+    const bool tagged = TAGGABLE && P.tagged != 0;
+    const int objOff = tagged ? 2 : 0;
+    unsigned tseq = 0;
     for (int f = threadIdx.x; f < BP; f += BLOCK) {
         const bool real = f < P.B;
         s.iter[f] = (P.resume && real) ? P.ctlIter[f] : 0;
@@ -1842,7 +2003,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     // initial objective -> lastObj
     if (!P.resume) {
         objectivePartial<Real, MODEL, BP>(P, objOp, s);
-        writeSlots<Real, BP>(P, seq, s, false);
+        writeSlots<Real, BP>(P, seq, s, false, objOff);
         if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
                 const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
                 cLast[f] = obj;
@@ -1850,17 +2011,19 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 cFlag[f] = 0;
                 cDone[f] = 0;
                 o[0] = 0.0;
-            }, false))
+            }, false, objOff))
             return;
     }
 
     double loc[4] = {0.0, 0.0, 0.0, 0.0};   // counters of workgroup 0, thread 0
     unsigned long long locSteps = 0;
     bool allDone = false;
+    int nActive = 0;   // fits still iterating this cycle
     {
         int act = 0;
-        for (int f = threadIdx.x; f < P.B; f += BLOCK) act |= s.active[f];
-        allDone = !__syncthreads_or(act);
+        for (int f = threadIdx.x; f < P.B; f += BLOCK) act += s.active[f];
+        nActive = __syncthreads_count(act != 0);
+        allDone = nActive == 0;
     }
     // coordinates in cycle order: non-fixed columns; the metadata of the next one
     // (and the beta/bound of the fits this workgroup steps) is fetched while the
@@ -1930,7 +2093,10 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             // prefetch for the next coordinate (a different column, so its beta and
             // bound are final for this cycle)
             const Meta nxt = loadMeta(P, jn, w);
-            havePre = jn != j;
+            // not across the end of the cycle: the next cycle's first beta and bound are
+            // read after the objective exchange, which orders them after workgroup 0's
+            // stores (tagged exchanges carry no such ordering)
+            havePre = jn > j;
             if (havePre && mine(threadIdx.x)) {
                 preB = ald(&P.beta[(size_t)threadIdx.x * P.J + jn]);
                 preH = ald(&P.hDelta[(size_t)threadIdx.x * P.J + jn]);
@@ -1962,7 +2128,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 prefetchRows();
                 __syncthreads();
             } else if constexpr (SWEEP) {
-                const unsigned long long tA = now();
+                const unsigned long long tA = P.timing ? now() : 0ull;
                 int64_t n0, n1;   // this thread's run-aligned chunk of the column
                 switch (fm) {
                 case F_INDICATOR: subChunk<Real, F_INDICATOR, true>(P, cur.a, cur.b, threadIdx.x, BLOCK, n0, n1); break;
@@ -2007,13 +2173,13 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     carryPending = false;
                 }
                 sweepWalk<Real, BP>(P, s, carryD, two);
-                const unsigned long long tB = now();
+                const unsigned long long tB = P.timing ? now() : 0ull;
 
                 double g = 0.0, h = 0.0;
                 if (!sweepReduce<Real, BP>(P, seq, deadline, s, g, h)) return;
                 if (threadIdx.x == 0) step(0, Mon{g, h, 0.0, 0.0, 0.0, 0.0}, s.out);
                 __syncthreads();
-                const unsigned long long tC = now();
+                const unsigned long long tC = P.timing ? now() : 0ull;
                 scatter(true);   // this workgroup's walk is over: numerators back to zero
                 double dl[1] = {s.out[0]};
                 if (dl[0] != 0.0) {
@@ -2036,7 +2202,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     carryPending = true;
                 }
                 if (w == 0 && threadIdx.x == 0) {
-                    const unsigned long long tD = now();
+                    const unsigned long long tD = P.timing ? now() : 0ull;
                     loc[1] += (double)(tB - tA);
                     loc[2] += (double)(tC - tB);
                     loc[3] += (double)(tD - tC);
@@ -2055,7 +2221,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     loc[0] += by;
                 }
             } else {
-                const unsigned long long tA = now();
+                const unsigned long long tA = P.timing ? now() : 0ull;
                 const int sg = threadIdx.x / GE::SUB, sl = threadIdx.x % GE::SUB;
                 int64_t n0, n1;
                 switch (fm) {
@@ -2116,22 +2282,24 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 } else {
                     combinePlain<BP, GE>(acc, s.mon);
                 }
-                const unsigned long long tB = now();
-                writeSlots<Real, BP>(P, seq, s, GROUPED);
-                if (!reduceFits<Real, BP>(P, seq, deadline, s, step, GROUPED)) return;
-                const unsigned long long tC = now();
+                const unsigned long long tB = P.timing ? now() : 0ull;
+                if (tagged) {
+                    writeTagged<Real, BP>(P, tseq, s);
+                    if (!reduceTagged<Real, BP>(P, tseq, deadline, s, step)) return;
+                } else {
+                    writeSlots<Real, BP>(P, seq, s, GROUPED);
+                    if (!reduceFits<Real, BP>(P, seq, deadline, s, step, GROUPED)) return;
+                }
+                const unsigned long long tC = P.timing ? now() : 0ull;
                 double dl[GE::FPL];
-                int any = 0;
 #pragma unroll
                 for (int i = 0; i < GE::FPL; ++i) {
                     const int f = GE::fit(sl, i);
                     dl[i] = f < P.B ? s.out[4 * f] : 0.0;
-                    any |= (dl[i] != 0.0);
                 }
-                int nUpd = 0;
-                if (w == 0 && threadIdx.x == 0)
-                    for (int f = 0; f < P.B; ++f) nUpd += (s.out[4 * f] != 0.0) ? 1 : 0;
-                if (__syncthreads_or(any)) {
+                // fits that moved (B <= BP <= BLOCK: thread f looks at fit f)
+                const int nUpd = __syncthreads_count(threadIdx.x < P.B && s.out[4 * threadIdx.x] != 0.0);
+                if (nUpd > 0) {
                     if (FAST && fast) {
                         applyBatch<Real, MODEL, BP, UF, LOW>(P, bt, dl);
                     } else {
@@ -2148,7 +2316,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     }
                 }
                 if (w == 0 && threadIdx.x == 0) {
-                    const unsigned long long tD = now();
+                    const unsigned long long tD = P.timing ? now() : 0ull;
                     loc[1] += (double)(tB - tA);
                     loc[2] += (double)(tC - tB);
                     loc[3] += (double)(tD - tC);
@@ -2156,8 +2324,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     // compulsory bytes of this coordinate: the column pass reads ids /
                     // values / weights once and xb of every active fit; the update reads
                     // and writes xb of every fit that moved
-                    double nAct = 0;
-                    for (int f = 0; f < P.B; ++f) nAct += s.active[f] ? 1 : 0;
+                    const double nAct = (double)nActive;
                     const double n = (double)nnzj;
                     const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
                                              ((fm == F_SPARSE || fm == F_DENSE) ? sizeof(Real) : 0.0) +
@@ -2178,7 +2345,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             if (jn <= j) break;   // wrapped: the cycle is complete
         }
         // convergence test of every fit (performCheckConvergence :839-906)
-        const unsigned long long tO = now();
+        const unsigned long long tO = P.timing ? now() : 0ull;
         if constexpr (SWEEP) {   // D of every pid for the objective (and exact wave totals)
             if (carryPending) {   // the last update's exchange, joined so the sequence stays in step
                 if (!sweepCarryWait<Real, BP>(P, seq, deadline, s, carryD)) return;
@@ -2188,7 +2355,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             denomStale = true;
         }
         objectivePartial<Real, MODEL, BP>(P, objOp, s);
-        writeSlots<Real, BP>(P, seq, s, false);
+        writeSlots<Real, BP>(P, seq, s, false, objOff);
         if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
                 if (cDone[f]) { o[0] = 1.0; return; }
                 const double obj = m.g + (addFixed ? P.fixedTermFit[f] : 0.0) + (objOp == OBJ_LANGE ? logPrior(f) : 0.0);
@@ -2203,7 +2370,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 else if (cIter[f] == P.maxIt) { cFlag[f] = CCD_MAX_ITERATIONS; done = 1; }
                 cDone[f] = done;
                 o[0] = done;
-            }, false))
+            }, false, objOff))
             return;
         int notDone = 0;
         for (int f = threadIdx.x; f < P.B; f += BLOCK) {
@@ -2211,9 +2378,10 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             s.active[f] = act;
             notDone |= act;
         }
-        allDone = !__syncthreads_or(notDone);
+        nActive = __syncthreads_count(notDone);
+        allDone = nActive == 0;
         if (w == 0 && threadIdx.x == 0) {   // flush this cycle's counters (once per cycle, not per coordinate)
-            atomicAdd(&P.stats[4], (double)(now() - tO));
+            if (P.timing) atomicAdd(&P.stats[4], (double)(now() - tO));
             for (int i = 0; i < 4; ++i) atomicAdd(&P.stats[i], loc[i]);
             atomicAdd(P.steps, locSteps);
             loc[0] = loc[1] = loc[2] = loc[3] = 0.0;
@@ -2238,12 +2406,12 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     if (!allDone) return;
     // final log-likelihood of every fit (reported beside beta)
     objectivePartial<Real, MODEL, BP>(P, OBJ_LOGLIK, s);
-    writeSlots<Real, BP>(P, seq, s, false);
+    writeSlots<Real, BP>(P, seq, s, false, objOff);
     if (!reduceFits<Real, BP>(P, seq, deadline, s, [&](int f, const Mon& m, double* o) {
             o[0] = m.g + ((MODEL == M_POISSON) ? P.fixedTermFit[f] : 0.0);
             o[1] = cIter[f];
             o[2] = cFlag[f];
-        }, false))
+        }, false, objOff))
         return;
     if (w == 0)
         for (int f = threadIdx.x; f < P.B; f += BLOCK) {
