@@ -3,9 +3,10 @@
 plain per-fit GPU path, on the Poisson workload (1,000,000 rows, 20,000 Zipf(1.1)
 indicator columns at 2% mean density, log time-at-risk offset, Normal prior).
 
-The batch: B fits of that data from a cold start, 4 row folds (BY_ROW selection, each
-fit drops one fold) x B/4 prior variances on a log grid around 1.0, FP32 bulk
-arithmetic, convergence on relative log-likelihood change < 1e-6.
+The batch: B = 128 fits of that data from a cold start, 8 row folds (BY_ROW
+selection, each fit drops one fold) x 16 prior variances on a log grid from 0.1 to
+10, FP32 bulk arithmetic, convergence on relative log-likelihood change < 1e-6. 128
+fits fill the device: teams of 16 fits x 64 workgroups, two workgroups per CU.
 
 One step = one CCD cycle of every fit of the batch, through the stepped-batch
 interface (ccd_batch_step); the batch then steps on until every fit has converged,
@@ -15,7 +16,7 @@ one launch per engine call) and the single-fit resident kernel are timed on one
 member of the same mix over a fixed number of cycles and extrapolated to the cycles
 that member needs; the same member is refitted in FP64 to state the FP32 error.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--batch 64]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--batch 128] [--folds 8]
 """
 import argparse
 import json
@@ -30,12 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-# HBM bytes per one-cycle launch of the default batch (64 fits, teams of 16 fits x 64
-# workgroups, full size): two rocprofv3 --pmc passes over `--workload traffic`,
+# HBM bytes per one-cycle launch of the headline batch (full size, teams of 16 fits x
+# 64 workgroups), by batch size: two rocprofv3 --pmc passes over `--workload traffic`,
 # FETCH_SIZE x 2 plus WRITE_SIZE x 1 (the factors the copy kernel over 1 GiB reads
 # back); the commands and per-dispatch values are in
-# profiles/kFindMode_poisson_K1e6_J2e4_batch64_traffic.md.
-TRAFFIC_BYTES_PER_CYCLE_B64 = 6.169e11
+# profiles/kFindMode_poisson_K1e6_J2e4_batch{64,128}_traffic.md.
+TRAFFIC_BYTES_PER_CYCLE = {64: 6.169e11, 128: 1.410e12}
 
 
 def log(msg):
@@ -67,15 +68,41 @@ def poisson_cycle_bytes(d, s, fits):
     return nnz * (8.0 + 7.0 * s * fits) + 4.0 * s * d.K * fits
 
 
+def cox_cycle_bytes(d, s, fits):
+    """Algorithmic bytes of one CCD cycle of `fits` Breslow Cox fits by the survey's rule,
+    counting every coordinate's numerator pass nnz_j (8 + 3 s) + |S_j| (4 + s) and its
+    risk-set sweep (N - i0_j) 2 s + |S_j| (4 + s), with N the tie-group positions, S_j
+    the positions column j touches and i0_j the first of them. Coordinates whose beta
+    moves add nnz_j (8 + 6 s) + 2 s N each; they are not counted, so this is a lower
+    bound. DENSE columns touch every position."""
+    pid = np.zeros(d.K, dtype=np.int64)
+    tie = (d.y[1:] == 1.0) & (d.y[:-1] == 1.0) & (d.time[1:] == d.time[:-1])
+    pid[1:] = np.cumsum(~tie)
+    N = int(pid[-1]) + 1
+    total = 0.0
+    for j in range(d.J):
+        f = int(d.formats[j])
+        if f in (1, 2):   # SPARSE / INDICATOR
+            a, b = int(d.col_ptr[j]), int(d.col_ptr[j + 1])
+            if b == a:
+                continue
+            pj = pid[d.rows[a:b]]
+            nnz, S, i0 = b - a, int(np.count_nonzero(np.diff(pj))) + 1, int(pj[0])
+        else:
+            nnz, S, i0 = d.K, N, 0
+        total += nnz * (8 + 3 * s) + 2 * S * (4 + s) + (N - i0) * 2 * s
+    return total * fits, N
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (one CCD cycle of the whole batch each)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps before the timed ones")
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--plain-cycles", type=int, default=12,
                     help="CCD cycles over which the plain per-fit path is timed (its per-cycle cost is constant)")
-    ap.add_argument("--folds", type=int, default=4)
+    ap.add_argument("--folds", type=int, default=8)
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (1.0 = full size)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -92,7 +119,12 @@ def main():
     ap.add_argument("--max-rows", type=int, default=100_000, help="serving: largest request (rows)")
     ap.add_argument("--max-cols", type=int, default=10_000, help="serving: largest request (columns)")
     ap.add_argument("--slots", type=int, default=16, help="serving: concurrent fits")
-    ap.add_argument("--plain-requests", type=int, default=4, help="serving: requests timed on the plain queue")
+    ap.add_argument("--pack", type=int, default=0,
+                    help="serving: queued one-workgroup requests of one model that may share a launch (0: none)")
+    ap.add_argument("--plain-requests", type=int, default=4,
+                    help="serving: first requests also run on a real plain-mode server (checks the plain-queue model)")
+    ap.add_argument("--cpu-sample-nnz", type=float, default=2e8,
+                    help="serving: largest request whose whole CPU cycle is timed (larger ones: first coordinates)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -208,9 +240,9 @@ def run_poisson(args):
     achieved = algo_bytes / (launch_ms / 1e3) / 1e9
     kernel_bytes = ks["algorithmic_bytes"] / max(ks["launches"], 1)
     # counter-measured traffic applies to the shape it was measured on
-    measured_shape = (B == 64 and args.scale == 1.0 and lay["fits_per_team"] == 16 and
-                      lay["workgroups_per_team"] == 64 and min(active_per_step) == B)
-    traffic = TRAFFIC_BYTES_PER_CYCLE_B64 if measured_shape else None
+    measured_shape = (B in TRAFFIC_BYTES_PER_CYCLE and folds == B // 16 and args.scale == 1.0 and
+                      lay["fits_per_team"] == 16 and lay["workgroups_per_team"] == 64 and min(active_per_step) == B)
+    traffic = TRAFFIC_BYTES_PER_CYCLE[B] if measured_shape else None
 
     # one member of the mix: variance closest to 1.0, fold 0
     ref_i = int(np.argmin([abs(math.log(v)) + 10 * h for v, h in zip(var_list, held)]))
@@ -364,24 +396,33 @@ def run_poisson(args):
     print(json.dumps(out), flush=True)
 
 
-def _serve(E, reqs, arrivals, slots, mode, budget):
+def _serve(E, reqs, arrivals, slots, mode, budget, pack=0):
     """Submit reqs at the given arrival offsets (seconds) to a fresh server; returns
     per-request (submit, start, end) seconds on the server's clock and the results."""
     import threading
     from cyclops_amd.control import create_control, create_prior
-    srv = E.Server(slots=slots, hbm_budget_bytes=budget, mode=mode)
+    srv = E.Server(slots=slots, hbm_budget_bytes=budget, mode=mode, pack=pack)
     ctl = create_control(convergence_type="mittal", tolerance=1e-6)
     ids = [None] * len(reqs)
     out = [None] * len(reqs)
 
     def collect():
-        for i in range(len(reqs)):
-            while ids[i] is None:
-                time.sleep(0.001)
-            out[i] = srv.poll(ids[i], wait=True)
-            s = out[i][2]
-            log(f"{mode} request {i} ({reqs[i]['model']} K={reqs[i]['data'].K} J={reqs[i]['data'].J}): "
-                f"{out[i][1]['iterations']} cycles, latency {s['end'] - s['submit']:.2f}s")
+        # results in the order they finish (latencies come from the server's clock)
+        pending = list(range(len(reqs)))
+        while pending:
+            left = []
+            for i in pending:
+                got = srv.poll(ids[i], wait=False) if ids[i] is not None else None
+                if got is None:
+                    left.append(i)
+                    continue
+                out[i] = got
+                s = got[2]
+                log(f"{mode} request {i} ({reqs[i]['model']} K={reqs[i]['data'].K} J={reqs[i]['data'].J}): "
+                    f"{got[1]['iterations']} cycles, latency {s['end'] - s['submit']:.2f}s")
+            if len(left) == len(pending):
+                time.sleep(0.01)
+            pending = left
 
     th = threading.Thread(target=collect)
     th.start()
@@ -396,58 +437,166 @@ def _serve(E, reqs, arrivals, slots, mode, budget):
     return out
 
 
+def _queue_latencies(arrivals, service, workers):
+    """FIFO queue with `workers` identical servers: each request starts when it has
+    arrived and a server is free, in arrival order. Returns per-request (start, end)."""
+    free = [0.0] * workers
+    start = np.zeros(len(service))
+    end = np.zeros(len(service))
+    for i in np.argsort(arrivals, kind="stable"):
+        k = int(np.argmin(free))
+        start[i] = max(arrivals[i], free[k])
+        end[i] = start[i] + service[i]
+        free[k] = end[i]
+    return start, end
+
+
+def _queue_summary(arrivals, end, what):
+    lat = end - arrivals
+    span = float(np.max(end) - np.min(arrivals))
+    return {"fits_per_sec": len(end) / span, "span_s": span, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "how": what}
+
+
 def run_serving(args):
-    """Serving mix: LR / PR / Cox requests (0.5 / 0.3 / 0.2) of log-uniform size, each
-    generated from its own seed, arriving as a Poisson process; the continuous-batching
-    server (resident fits, `slots` concurrent, 200 GB HBM budget) against the plain
-    per-fit queue (one fit at a time through the per-coordinate entry points)."""
+    """Serving mix: LR / PR / Cox requests (0.5 / 0.3 / 0.2) with rows log-uniform in
+    [1e4, max_rows] and columns log-uniform in [1e3, max_cols], each generated from its
+    own seed, arriving as a Poisson process; the continuous-batching server (resident
+    fits, `slots` concurrent, 200 GB HBM budget) against the plain per-fit queue (one
+    fit at a time through the per-coordinate entry points) and the CPU oracle on a
+    pool of host cores.
+
+    The plain queue and the CPU pool are timed per request on a sample of each fit
+    (the plain path: one CCD cycle of every request; the CPU: one cycle, or the first
+    coordinates of one for requests above --cpu-sample-nnz entries, on one core per
+    request, all host cores busy) and extrapolated to the cycles the served fit took;
+    the same arrivals then run through a FIFO queue of those service times (one
+    server for the plain path, one per host core for the CPU). The first
+    --plain-requests requests also run on a real plain-mode server, which checks the
+    model against measured latencies."""
     from concurrent.futures import ThreadPoolExecutor
     from cyclops_amd import _abi
     from cyclops_amd import data as D
     from cyclops_amd import engine as E
+    from cyclops_amd.control import create_control, create_prior
 
     # one hardware queue per slot, so every slot's persistent kernel can run at once
     # (set before the first HIP call of the process)
     os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, args.slots)))
+    # one stderr line per persistent launch of every request (progress of long fits)
+    os.environ.setdefault("CCD_PROGRESS", "1")
     if E.device_count() < 1:
         raise SystemExit("no HIP device")
+    model_name, ncpu = cpu_info()
+    workers = min(16, ncpu or 1)
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=8) as ex:
+    with ThreadPoolExecutor(max_workers=workers) as ex:
         reqs = list(ex.map(lambda r: D.make_request(r, max_rows=args.max_rows, max_cols=args.max_cols),
                            range(args.requests)))
     for r in reqs:
         r["precision"] = _abi.FP32
+    nnz = np.array([r["data"].nnz for r in reqs], dtype=np.float64)
     log(f"generated {len(reqs)} requests in {time.perf_counter() - t0:.1f}s "
-        f"(nnz {min(r['data'].nnz for r in reqs)}..{max(r['data'].nnz for r in reqs)})")
+        f"(nnz {int(nnz.min())}..{int(nnz.max())}, total {nnz.sum():.3g})")
     rng = np.random.default_rng(4242)
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(reqs)))
     arrivals -= arrivals[0]
     budget = 200e9
 
-    out = _serve(E, reqs, arrivals, args.slots, "resident", budget)
+    out = _serve(E, reqs, arrivals, args.slots, "resident", budget, pack=args.pack)
     st = [o[2] for o in out]
     lat = np.array([s["end"] - s["submit"] for s in st])
     span = max(s["end"] for s in st) - min(s["submit"] for s in st)
     flags = sorted({o[1]["return_flag"] for o in out})
+    cycles = np.array([o[1]["iterations"] for o in out], dtype=np.float64)
     log(f"served {len(reqs)} requests in {span:.2f}s: p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s")
 
-    plain = None
+    # ---- plain per-fit path: the first two cycles of every request, one at a time --
+    # (the first cycle moves far more coordinates than later ones under a Laplace
+    # prior, and for Cox each moved coordinate rescans the risk sets, so the later
+    # cycles are extrapolated from the second)
+    ctl1 = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=1)
+    ctl2 = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=2)
+    plain_setup = np.zeros(len(reqs))
+    plain_c1 = np.zeros(len(reqs))
+    plain_c2 = np.zeros(len(reqs))
+    t_meas = time.perf_counter()
+    for i, r in enumerate(reqs):
+        t0 = time.perf_counter()
+        eng = E.Engine(r["data"], _abi.FP32)
+        plain_setup[i] = time.perf_counter() - t0
+        prior = create_prior(r["prior"], r["variance"])
+        t0 = time.perf_counter()
+        eng.fit(prior, ctl1, mode="plain")
+        plain_c1[i] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, r2 = eng.fit(prior, ctl2, mode="plain")
+        t2 = time.perf_counter() - t0
+        plain_c2[i] = max(t2 - plain_c1[i], 0.0) if r2["iterations"] >= 2 else plain_c1[i]
+        eng.close()
+        log(f"plain request {i}: set-up {plain_setup[i]:.2f}s, cycle 1 {plain_c1[i]:.2f}s, cycle 2 {plain_c2[i]:.2f}s")
+    log(f"plain path: two cycles of each request in {time.perf_counter() - t_meas:.1f}s")
+    plain_service = plain_setup + plain_c1 + plain_c2 * np.maximum(cycles - 1.0, 0.0)
+    _, endp = _queue_latencies(arrivals, plain_service, 1)
+    plain = _queue_summary(arrivals, endp, "the first two CCD cycles of each request on the plain path (plus its "
+                                           "engine set-up), the later cycles extrapolated from the second up to the "
+                                           "cycles the served fit took, through one FIFO server with the same arrivals")
+    plain["service_seconds"] = float(plain_service.sum())
     m = min(args.plain_requests, len(reqs))
     if m > 0:
-        # the same first m requests, arriving at the same times, on the plain queue
+        # the model against a real plain-mode server on the first m requests
         outp = _serve(E, reqs[:m], arrivals[:m], 1, "plain", budget)
         stp = [o[2] for o in outp]
         latp = np.array([s["end"] - s["submit"] for s in stp])
-        spanp = max(s["end"] for s in stp) - min(s["submit"] for s in stp)
+        _, endm = _queue_latencies(arrivals[:m], plain_service[:m], 1)
         diff = max(float(np.max(np.abs(a[0] - b[0])) / max(np.max(np.abs(b[0])), 1e-12))
                    for a, b in zip(out[:m], outp))
-        svc = sum(s["end"] - s["start"] for s in stp)
-        svc_res = sum(s["end"] - s["start"] for s in st[:m])
-        plain = {"requests": m, "fits_per_sec": m / spanp, "p50_s": float(np.percentile(latp, 50)),
-                 "p99_s": float(np.percentile(latp, 99)), "service_seconds": svc,
-                 "served_service_seconds_same_requests": svc_res,
-                 "max_coef_rel_diff_vs_served": diff}
-        log(f"plain queue, first {m} requests: {m / spanp:.3f} fits/s, service {svc:.1f}s vs {svc_res:.1f}s served")
+        plain["check_first_requests"] = {
+            "requests": m, "measured_p50_s": float(np.percentile(latp, 50)),
+            "model_p50_s": float(np.percentile(endm - arrivals[:m], 50)),
+            "measured_service_s": float(sum(s["end"] - s["start"] for s in stp)),
+            "model_service_s": float(plain_service[:m].sum()),
+            "max_coef_rel_diff_vs_served": diff}
+        log(f"plain check, first {m}: measured service {plain['check_first_requests']['measured_service_s']:.1f}s, "
+            f"model {plain['check_first_requests']['model_service_s']:.1f}s")
+
+    # ---- CPU oracle: one core per request, every host core busy ------------------
+    cpu = None
+    if not args.skip_cpu:
+        import oracle
+        ctl = create_control(convergence_type="mittal", tolerance=1e-6)
+
+        def cpu_one(i):
+            r = reqs[i]
+            d = r["data"]
+            prior = create_prior(r["prior"], r["variance"])
+            # work of one cycle in entries: the column entries, plus a pass over the
+            # rows per coordinate for Cox (its risk-set sweep)
+            work = d.nnz + (d.K * (d.J - 1) if r["model"] == "cox" else 0)
+            if work <= args.cpu_sample_nnz:
+                # first and second cycle, later ones extrapolated from the second
+                c1, _ = oracle.time_cycles(d, prior, ctl, 1)
+                c12, _ = oracle.time_cycles(d, prior, ctl, 2)
+                return c1 + max(c12 - c1, 0.0) * max(cycles[i] - 1.0, 0.0)
+            # the first coordinates of the first cycle, extrapolated to every cycle (the
+            # first cycle moves the most coordinates: an upper bound on the CPU's time)
+            n = max(1, int((d.J - 1) * args.cpu_sample_nnz / work))
+            return oracle.time_coordinates(d, prior, ctl, n) / n * (d.J - 1) * cycles[i]
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            cpu_service = np.array(list(ex.map(cpu_one, range(len(reqs)))))
+        t_cpu = time.perf_counter() - t0
+        _, endc = _queue_latencies(arrivals, cpu_service, workers)
+        q = _queue_summary(arrivals, endc, "")
+        cpu = {"value": q["fits_per_sec"], "unit": "fits/s", "cores": workers, "kind": "port",
+               "cpu_model": model_name, "host_logical_cpus": ncpu, "p50_s": q["p50_s"], "p99_s": q["p99_s"],
+               "sample": f"the oracle (FP64) on {workers} host threads, one request per thread: the first two CCD "
+                         f"cycles of each request (later cycles extrapolated from the second), or for requests over "
+                         f"{args.cpu_sample_nnz:.0e} entries of work per cycle the first coordinates of the first "
+                         f"cycle ({t_cpu:.1f}s of wall time), up to the cycles the served fit took; the same arrivals "
+                         f"through a FIFO pool of {workers} workers"}
+        log(f"cpu pool: {cpu['value']:.4f} fits/s ({t_cpu:.1f}s of sampling)")
     res = {
         "metric": "fits/sec + p50/p99 fit latency vs plain per-fit GPU path at matched beta tol",
         "value": len(reqs) / span,
@@ -463,10 +612,15 @@ def run_serving(args):
         "data": "synthetic (one seeded generator per request, no dataset files)",
         "config": {"workload": f"serving_mix_{len(reqs)}req_{args.rate:g}rps_rows1e4-{args.max_rows}_cols1e3-{args.max_cols}",
                    "models": {k: sum(r["model"] == k for r in reqs) for k in ("lr", "pr", "cox")},
-                   "slots": args.slots, "hbm_budget_bytes": budget, "return_flags": flags},
+                   "rows_max": int(max(r["data"].K for r in reqs)), "cols_max": int(max(r["data"].J for r in reqs)),
+                   "nnz_total": float(nnz.sum()), "nnz_max": float(nnz.max()),
+                   "slots": args.slots, "pack": args.pack, "hbm_budget_bytes": budget, "return_flags": flags,
+                   "cycles_median": float(np.median(cycles)), "cycles_max": float(cycles.max())},
         "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3,
                        "note": "arrival to result, server clock"},
         "plain_queue": plain,
+        "speedup_vs_plain": (len(reqs) / span) / plain["fits_per_sec"],
+        "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
 
@@ -475,8 +629,9 @@ def run_traffic(args):
     """Short run for the HBM-traffic counters (rocprofv3 --pmc FETCH_SIZE, then
     WRITE_SIZE): the copy kernel of ccd_copy_bandwidth over 1 GiB buffers (two
     launches of known traffic, for scaling the counters), then three single-cycle
-    steps of the headline batch (64 Poisson fits, teams of 16 fits x 64 workgroups),
-    whose algorithmic bytes per cycle are printed for comparison."""
+    steps of the headline batch (--batch Poisson fits, --folds folds x 16 variances,
+    teams of 16 fits x 64 workgroups), whose algorithmic bytes per cycle are printed
+    for comparison."""
     from cyclops_amd import _abi
     from cyclops_amd import data as D
     from cyclops_amd import engine as E
@@ -487,10 +642,12 @@ def run_traffic(args):
     log(f"copy kernel: 2 launches, each reading {nbytes} bytes and writing {nbytes} bytes")
     os.environ["CCD_BATCH_LAYOUT"] = "16,64"
     d, _ = D.make_config(1, scale=args.scale)
-    rf = E.assign_folds(d.K, None, 4, _abi.SELECT_BY_ROW, 1001)
-    grid = np.exp(np.linspace(math.log(0.1), math.log(10.0), 16))
-    var_list = [float(v) for v in grid for _ in range(4)]
-    held = [f for _ in range(16) for f in range(4)]
+    folds = args.folds
+    n_var = max(1, args.batch // folds)
+    rf = E.assign_folds(d.K, None, folds, _abi.SELECT_BY_ROW, 1001)
+    grid = np.exp(np.linspace(math.log(0.1), math.log(10.0), n_var))
+    var_list = [float(v) for v in grid for _ in range(folds)]
+    held = [f for _ in range(n_var) for f in range(folds)]
     eng = E.Engine(d, _abi.FP32)
     run = eng.begin_batch(var_list, "normal", create_control(convergence_type="mittal", tolerance=1e-6),
                           held_out_fold=held, row_fold=rf)
@@ -602,7 +759,16 @@ def run_other(args):
                       f"({cpc:.3f}s per cycle), extrapolated to its {it_ref} cycles")
         cpu = {"value": 1.0 / (cpc * it_ref), "unit": "fits/s", "cores": 1, "kind": "port", "sample": sample}
     launch_ms = ks["device_ms"] / max(ks["launches"], 1)
-    achieved = (ks["algorithmic_bytes"] / max(ks["launches"], 1)) / (launch_ms / 1e3) / 1e9 if launch_ms > 0 else 0.0
+    kernel_gbs = (ks["algorithmic_bytes"] / max(ks["launches"], 1)) / (launch_ms / 1e3) / 1e9 if launch_ms > 0 else 0.0
+    achieved, bytes_rule, n_pos = kernel_gbs, "bytes the kernel counts (its own compulsory traffic)", None
+    if idx == 2:
+        # the survey's Cox rule (a lower bound: moved coordinates not counted) over the
+        # cycles the batch ran, against the batch's total kernel time
+        cyc_bytes, n_pos = cox_cycle_bytes(d, 4.0, B)
+        achieved = cyc_bytes * max(iters) / (ks["device_ms"] / 1e3) / 1e9
+        bytes_rule = ("survey rule per coordinate and fit: numerator nnz_j (8 + 3 s) + |S_j| (4 + s), sweep "
+                      "(N - i0_j) 2 s + |S_j| (4 + s), s = 4; coordinates that move (nnz_j (8 + 6 s) + 2 s N more) "
+                      "are not counted, so this is a lower bound")
     out = {
         "metric": "fits/sec + p50/p99 fit latency vs plain per-fit GPU path at matched beta tol",
         "value": B / step, "unit": "fits/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": step * 1e3,
@@ -619,7 +785,8 @@ def run_other(args):
         "resident_single_fit": {"seconds_per_cycle": res_pc, "fits_per_sec": 1.0 / (res_pc * it_ref)},
         "speedup_vs_plain": (B / step) * plain_pc * it_ref,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": launch_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": launch_ms,
+                     "bytes_rule": bytes_rule, "kernel_counted_gbs": kernel_gbs, "risk_set_positions": n_pos},
         "cpu_baseline": cpu,
     }
     if cv is not None:

@@ -143,7 +143,7 @@ class ServerConfig(C.Structure):
         ("slots", C.c_int32),
         ("hbm_budget_bytes", C.c_double),
         ("mode", C.c_int32),
-        ("pad_", C.c_int32),
+        ("pack", C.c_int32),
     ]
 
 

@@ -151,7 +151,7 @@ struct ResidentNeeds {
     double capacityFraction;
     double workspaceBytes;
 };
-ResidentNeeds residentNeeds(const ccd_data_desc& d, int device);
+ResidentNeeds residentNeeds(const ccd_data_desc& d, int device, int maxWorkgroups = 0);
 
 // Host CCD driver over an EngineBase: CyclicCoordinateDescent::update/findMode
 // (CyclicCoordinateDescent.cpp:586-615, 909-1216). This is the plain per-fit path.
@@ -195,5 +195,10 @@ struct Arena {
 };
 void setThreadArena(Arena* a);   // nullptr: plain hipMalloc / hipFree
 Arena* threadArena();
+// Most workgroups a single fit's persistent launch may take on the calling thread (0:
+// no limit beyond the device). The server sets it on its workers, so that no one
+// request holds the whole device's co-resident capacity.
+void setThreadMaxWorkgroups(int g);
+int threadMaxWorkgroups();
 
 }  // namespace ccd

@@ -23,6 +23,9 @@ namespace ccd {
 static thread_local Arena* g_arena = nullptr;
 void setThreadArena(Arena* a) { g_arena = a; }
 Arena* threadArena() { return g_arena; }
+static thread_local int g_maxWG = 0;
+void setThreadMaxWorkgroups(int g) { g_maxWG = g > 0 ? g : 0; }
+int threadMaxWorkgroups() { return g_maxWG; }
 
 static bool inArena(const void* p) {
     const char* c = static_cast<const char*>(p);
@@ -816,9 +819,10 @@ public:
 
 namespace ccd {
 
-ResidentNeeds residentNeeds(const ccd_data_desc& d, int device) {
+ResidentNeeds residentNeeds(const ccd_data_desc& d, int device, int maxWorkgroups) {
     ResidentNeeds n{};
-    const int G = chooseWorkgroups(d.K, 1, device, d.model_type);
+    int G = chooseWorkgroups(d.K, 1, device, d.model_type);
+    if (maxWorkgroups > 0 && G > maxWorkgroups) G = maxWorkgroups;
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const int occ = precisionOf(d.precision) == CCD_FP32 ? occupancyOf<float>(d.model_type, 1) : occupancyOf<double>(d.model_type, 1);

@@ -239,7 +239,8 @@ static int chooseWorkgroups(int64_t K, int B, int device, int model) {
     const bool grouped = model == M_CLR || model == M_COX;
     int cus = 0;
     CCD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    const int cap = std::min(res::MAXG, cus);
+    int cap = std::min(res::MAXG, cus);
+    if (B <= 2 && threadMaxWorkgroups() > 0) cap = std::min(cap, threadMaxWorkgroups());
     if (const char* e = std::getenv("CCD_RESIDENT_WG")) {
         const int v = std::atoi(e);
         if (v > 0) return std::min(v, cap);

@@ -44,6 +44,9 @@ struct Request {
     double footprint = 0.0;
     double capFraction = 0.0;
     size_t blockOff = 0, blockSize = 0;
+    size_t heldOff = 0, heldSize = 0;   // the region block this request frees when done (a packed group's first member holds the group's block)
+    int model = 0, prec = 0;
+    bool packable = false;   // resident fit of one workgroup: may share a launch with others of its model and precision
     // outcome
     ReqState state = QUEUED;
     int rc = CCD_OK;
@@ -116,23 +119,96 @@ struct ccd_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
+    // A request's persistent launch takes at most a quarter of the device's CUs in
+    // workgroups, so that no single large fit (a Cox fit asks for one workgroup per
+    // 1,024 rows) holds the whole co-resident capacity while others queue behind it.
+    int maxWorkgroups() const {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg.device) != hipSuccess) return 0;
+        return std::max(1, cus / 4);
+    }
+
     // First queued request that fits the remaining budget (caller holds m). Later
-    // requests may overtake one that does not fit yet, until the oldest has waited
-    // 2 s: then nothing overtakes it (a large request cannot starve).
+    // requests may overtake one that does not fit yet. Once the oldest has waited 2 s
+    // it holds a reservation: a later request starts only if the co-resident capacity
+    // and an HBM block for the oldest would still be left, so overtakers never take
+    // what the oldest is waiting for (a large request cannot starve, and small ones
+    // keep flowing past it).
     std::shared_ptr<Request> pick() {
-        const bool headWaited = !queue.empty() && now() - queue.front()->stats.submit_time > 2.0;
+        if (queue.empty()) return nullptr;
+        const auto& head = queue.front();
+        const bool headWaited = now() - head->stats.submit_time > 2.0;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
-            if (headWaited && it != queue.begin()) break;
             const auto& r = *it;
-            if (usedCap + r->capFraction > 1.0 + 1e-9) continue;
+            const bool reserve = headWaited && it != queue.begin();
+            if (usedCap + r->capFraction + (reserve ? head->capFraction : 0.0) > 1.0 + 1e-9) continue;
             size_t off = 0;
             if (!region.alloc(r->blockSize, off)) continue;
+            if (reserve) {
+                size_t hoff = 0;
+                if (!region.alloc(head->blockSize, hoff)) {
+                    region.release(off, r->blockSize);
+                    continue;
+                }
+                region.release(hoff, head->blockSize);
+            }
             r->blockOff = off;
             auto q = r;
             queue.erase(it);
             return q;
         }
         return nullptr;
+    }
+
+    // The admitted request r plus up to cfg.pack - 1 later queued requests of its model
+    // and precision whose resident fit is one workgroup (caller holds m). The group
+    // takes one region block, the sum of its members' blocks: its engines carve their
+    // memory from it one after another, so a packed launch calls no hipMalloc either.
+    std::vector<std::shared_ptr<Request>> pickGroup() {
+        std::vector<std::shared_ptr<Request>> g;
+        auto r = pick();
+        if (!r) return g;
+        g.push_back(r);
+        if (cfg.pack <= 1 || cfg.mode != CCD_MODE_RESIDENT || !r->packable) return g;
+        size_t total = r->blockSize;
+        double cap = usedCap + r->capFraction;
+        // a waiting oldest request keeps its reserved share (see pick)
+        if (!queue.empty() && now() - queue.front()->stats.submit_time > 2.0) cap += queue.front()->capFraction;
+        std::vector<std::deque<std::shared_ptr<Request>>::iterator> take;
+        for (auto it = queue.begin(); it != queue.end() && (int)(g.size() + take.size()) < cfg.pack; ++it) {
+            const auto& q = *it;
+            if (!q->packable || q->model != r->model || q->prec != r->prec) continue;
+            if (cap + q->capFraction > 1.0 + 1e-9) continue;
+            cap += q->capFraction;
+            total += q->blockSize;
+            take.push_back(it);
+        }
+        if (take.empty()) return g;
+        // one block for the group, in place of r's own (and, for a waiting oldest
+        // request, only if its block would still fit beside it)
+        size_t off = 0;
+        region.release(r->blockOff, r->blockSize);
+        bool ok = region.alloc(total, off);
+        if (ok && !queue.empty() && now() - queue.front()->stats.submit_time > 2.0) {
+            size_t hoff = 0;
+            if (region.alloc(queue.front()->blockSize, hoff)) region.release(hoff, queue.front()->blockSize);
+            else {
+                region.release(off, total);
+                ok = false;
+            }
+        }
+        if (!ok) {
+            region.alloc(r->blockSize, off);   // the block just released fits again
+            r->blockOff = off;
+            return g;
+        }
+        r->blockOff = off;
+        for (auto it : take) g.push_back(*it);
+        for (auto k = take.rbegin(); k != take.rend(); ++k) queue.erase(*k);
+        r->heldOff = off;
+        r->heldSize = total;
+        for (size_t i = 1; i < g.size(); ++i) g[i]->heldSize = 0;
+        return g;
     }
 
     // A request's engine carves its device memory from the request's block of the
@@ -166,6 +242,7 @@ struct ccd_server {
         bool ready = true;
         try {
             CCD_HIP(hipSetDevice(cfg.device));
+            ccd::setThreadMaxWorkgroups(maxWorkgroups());
             CCD_HIP(hipStreamCreateWithFlags(&arena.stream, hipStreamNonBlocking));
             CCD_HIP(hipHostMalloc(reinterpret_cast<void**>(&arena.host), 64 * sizeof(double), hipHostMallocDefault));
             ccd::setThreadArena(&arena);
@@ -179,41 +256,90 @@ struct ccd_server {
         if (arena.stream) (void)hipStreamDestroy(arena.stream);
     }
 
+    // Requests packed into one launch: one engine each, carved one after another from
+    // the group's block, then ccd_fit_multi over all of them (each fit keeps the team
+    // shape of its own resident launch, so its result equals that launch's).
+    void runGroup(std::vector<std::shared_ptr<Request>>& g, ccd::Arena& arena, bool useArena) {
+        CCD_HIP(hipSetDevice(cfg.device));
+        if (useArena) {
+            arena.base = region.base + g[0]->blockOff;
+            arena.cap = g[0]->heldSize;
+            arena.used = 0;
+        }
+        const size_t n = g.size();
+        std::vector<ccd_handle> h(n, nullptr);
+        std::vector<ccd_fit_args> args(n);
+        std::vector<double*> betas(n);
+        std::vector<ccd_fit_result> res(n);
+        int rc = CCD_OK;
+        for (size_t i = 0; i < n && rc == CCD_OK; ++i) {
+            Request& r = *g[i];
+            rc = ccd_create(&r.desc, cfg.device, &h[i]);
+            if (rc == CCD_OK) {
+                std::vector<int32_t>().swap(r.rows);
+                std::vector<double>().swap(r.values);
+                std::vector<double>().swap(r.dense);
+            }
+            args[i] = r.args;
+            betas[i] = r.beta.data();
+        }
+        if (rc == CCD_OK) rc = ccd_fit_multi(h.data(), (int32_t)n, args.data(), betas.data(), res.data());
+        const std::string err = rc == CCD_OK ? std::string() : std::string(ccd_last_error());
+        for (size_t i = 0; i < n; ++i) {
+            if (h[i]) ccd_destroy(h[i]);
+            g[i]->rc = rc;
+            g[i]->error = err;
+            if (rc == CCD_OK) g[i]->result = res[i];
+        }
+        arena.base = nullptr;
+        arena.cap = arena.used = 0;
+    }
+
     void loop(ccd::Arena& arena, bool useArena) {
         for (;;) {
-            std::shared_ptr<Request> r;
+            std::vector<std::shared_ptr<Request>> g;
             {
                 std::unique_lock<std::mutex> lk(m);
-                work.wait(lk, [&] { return (r = pick()) != nullptr || (stopping && queue.empty()); });
-                if (!r) return;
-                usedCap += r->capFraction;
-                r->state = RUNNING;
-                r->stats.start_time = now();
+                work.wait(lk, [&] { return !(g = pickGroup()).empty() || (stopping && queue.empty()); });
+                if (g.empty()) return;
+                const double t = now();
+                for (auto& r : g) {
+                    usedCap += r->capFraction;
+                    r->state = RUNNING;
+                    r->stats.start_time = t;
+                }
             }
             try {
-                run(*r, arena, useArena);
+                if (g.size() == 1) run(*g[0], arena, useArena);
+                else runGroup(g, arena, useArena);
             } catch (const std::exception& ex) {
-                r->rc = CCD_ERR_HIP;
-                r->error = ex.what();
+                for (auto& r : g) {
+                    r->rc = CCD_ERR_HIP;
+                    r->error = ex.what();
+                }
             }
             {
                 std::lock_guard<std::mutex> lk(m);
-                region.release(r->blockOff, r->blockSize);
-                usedCap -= r->capFraction;
-                r->stats.end_time = now();
-                r->state = DONE;
-                // keep only what poll returns
-                std::vector<int32_t>().swap(r->format);
-                std::vector<int64_t>().swap(r->colPtr);
-                std::vector<int64_t>().swap(r->densePtr);
-                std::vector<double>().swap(r->y);
-                std::vector<double>().swap(r->time);
-                std::vector<int32_t>().swap(r->pid);
-                std::vector<double>().swap(r->weights);
-                std::vector<uint8_t>().swap(r->exclude);
-                std::vector<uint8_t>().swap(r->fixed);
-                r->args.exclude = r->args.fixed = nullptr;
-                r->args.weights = nullptr;
+                if (g.size() == 1) region.release(g[0]->blockOff, g[0]->blockSize);
+                else region.release(g[0]->heldOff, g[0]->heldSize);
+                const double t = now();
+                for (auto& r : g) {
+                    usedCap -= r->capFraction;
+                    r->stats.end_time = t;
+                    r->state = DONE;
+                    // keep only what poll returns
+                    std::vector<int32_t>().swap(r->format);
+                    std::vector<int64_t>().swap(r->colPtr);
+                    std::vector<int64_t>().swap(r->densePtr);
+                    std::vector<double>().swap(r->y);
+                    std::vector<double>().swap(r->time);
+                    std::vector<int32_t>().swap(r->pid);
+                    std::vector<double>().swap(r->weights);
+                    std::vector<uint8_t>().swap(r->exclude);
+                    std::vector<uint8_t>().swap(r->fixed);
+                    r->args.exclude = r->args.fixed = nullptr;
+                    r->args.weights = nullptr;
+                }
             }
             done.notify_all();
             work.notify_all();
@@ -254,6 +380,7 @@ int ccd_server_start(const ccd_server_config* cfg, ccd_server_handle* out) {
         if (cfg->slots < 1 || cfg->slots > 64) throw std::invalid_argument("slots must be in [1, 64]");
         if (!(cfg->hbm_budget_bytes > 0)) throw std::invalid_argument("hbm_budget_bytes must be > 0");
         if (cfg->mode != CCD_MODE_PLAIN && cfg->mode != CCD_MODE_RESIDENT) throw std::invalid_argument("bad mode");
+        if (cfg->pack < 0 || cfg->pack > 64) throw std::invalid_argument("pack must be in [0, 64]");
         auto* s = new ccd_server();
         s->cfg = *cfg;
         s->t0 = std::chrono::steady_clock::now();
@@ -325,10 +452,13 @@ int ccd_server_submit(ccd_server_handle s, const ccd_data_desc* d, const ccd_fit
         // admission footprint: the engine plus the resident workspace
         double bytes = 0.0;
         if (ccd_estimate_footprint(&r->desc, &bytes) != CCD_OK) throw std::invalid_argument(ccd_last_error());
+        r->model = d->model_type;
+        r->prec = ccd::precisionOf(d->precision);
         if (s->cfg.mode == CCD_MODE_RESIDENT) {
-            const ccd::ResidentNeeds n = ccd::residentNeeds(r->desc, s->cfg.device);
+            const ccd::ResidentNeeds n = ccd::residentNeeds(r->desc, s->cfg.device, s->maxWorkgroups());
             bytes += n.workspaceBytes;
             r->capFraction = n.capacityFraction;
+            r->packable = n.workgroups == 1;
             // starting values on more than 32 SPARSE/INDICATOR columns: the fit forms xb
             // from a row-major copy of those columns (row pointers, column ids, values)
             int live = 0;

@@ -530,12 +530,13 @@ class Server:
     by HBM budget and run concurrently on the device; poll() returns each result
     with its submit / start / end times."""
 
-    def __init__(self, slots=4, hbm_budget_bytes=200e9, mode="resident", device=0):
+    def __init__(self, slots=4, hbm_budget_bytes=200e9, mode="resident", device=0, pack=0):
         cfg = _abi.ServerConfig()
         cfg.device = device
         cfg.slots = slots
         cfg.hbm_budget_bytes = float(hbm_budget_bytes)
         cfg.mode = {"plain": _abi.MODE_PLAIN, "resident": _abi.MODE_RESIDENT}[mode]
+        cfg.pack = int(pack)
         h = C.c_void_p()
         check(lib().ccd_server_start(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -357,7 +357,10 @@ typedef struct ccd_server_config {
     int32_t slots;            /* concurrent fits on the device */
     double  hbm_budget_bytes; /* admission budget */
     int32_t mode;             /* CCD_MODE_RESIDENT (default) or CCD_MODE_PLAIN */
-    int32_t pad_;
+    int32_t pack;             /* 0 or 1: one request per launch. n > 1 (resident mode): up to n
+                                 queued requests of one model and precision whose resident fit is
+                                 one workgroup share a persistent launch (ccd_fit_multi); each
+                                 result equals its own launch's, and the members finish together */
 } ccd_server_config;
 typedef struct ccd_request_stats {
     double submit_time;       /* seconds since server start */

@@ -181,3 +181,39 @@ def test_finished_requests_are_released(E):
         srv.poll(quick, wait=True)
     srv.poll(slow, wait=True)
     srv.stop()
+
+
+def test_packed_requests_equal_their_own_launches(E):
+    """With pack > 1, queued requests of one model and precision whose resident fit is
+    one workgroup share a persistent launch (ccd_fit_multi). Each result equals the
+    result of its own launch bit for bit, members of a group start and end together,
+    and a request of another model is never packed with them."""
+    ctl = create_control(convergence_type="mittal", tolerance=1e-8)
+    # a fit of 300 cycles (no convergence test) that holds the one slot for about a second
+    long_ctl = create_control(convergence_type="mittal", tolerance=0.0, max_iterations=300)
+    blocker = (D.make_request(2, max_rows=100_000, max_cols=10_000)["data"], create_prior("laplace", 0.1), long_ctl)
+    reqs = [(_cases.logistic(K=500 + 300 * i, seed=60 + i), create_prior("laplace", 0.5 + 0.1 * i), ctl)
+            for i in range(6)]
+    reqs.insert(3, (_cases.poisson(K=900, seed=70), create_prior("normal", 1.0), ctl))
+    print(f"[case] server packing: {len(reqs)} small requests behind a blocker, pack 4, one slot", flush=True)
+    srv = E.Server(slots=1, hbm_budget_bytes=8e9)
+    alone = [srv.poll(srv.submit(d, p, c), wait=True) for d, p, c in reqs]
+    srv.stop()
+    srv = E.Server(slots=1, hbm_budget_bytes=8e9, pack=4)
+    first = srv.submit(*blocker)   # holds the slot while the others queue
+    ids = [srv.submit(d, p, c) for d, p, c in reqs]
+    packed = [srv.poll(i, wait=True) for i in ids]
+    srv.poll(first, wait=True)
+    srv.stop()
+    for i, ((b1, r1, _), (b2, r2, _)) in enumerate(zip(alone, packed)):
+        assert np.array_equal(b1, b2), i
+        assert (r1["iterations"], r1["return_flag"], r1["log_likelihood"]) == \
+               (r2["iterations"], r2["return_flag"], r2["log_likelihood"]), i
+    starts = [round(p[2]["start"], 9) for p in packed]
+    lr_starts = [s for s, (d, _, _) in zip(starts, reqs) if d.model_type == _abi.LOGISTIC]
+    assert max(lr_starts.count(s) for s in lr_starts) >= 2, starts   # some logistic requests shared a launch
+    pr_start = starts[3]
+    assert pr_start not in lr_starts, starts                          # the Poisson one ran on its own
+    for s in set(lr_starts):
+        ends = {round(p[2]["end"], 9) for p, st in zip(packed, starts) if st == s}
+        assert len(ends) == 1
