@@ -753,6 +753,26 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
     }
 }
 
+// First half of combinePlain: the butterfly over the lanes of a wave that hold the
+// same fit, then wave w's sums in sMon[w * BP + f]. One-workgroup teams of
+// independent rows ("solo") add the waves themselves, in combinePlain's order.
+template <int BP, class GE>
+__device__ __forceinline__ void combineWaves(const Mon (&acc)[GE::FPL], Mon* sMon) {
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int sl = threadIdx.x % GE::SUB;
+#pragma unroll
+    for (int i = 0; i < GE::FPL; ++i) {
+        double g = acc[i].g, h = acc[i].h;
+#pragma unroll
+        for (int off = GE::SUB; off < 64; off <<= 1) {
+            g += __shfl_xor(g, off, 64);
+            h += __shfl_xor(h, off, 64);
+        }
+        if (lane < GE::SUB) sMon[wave * BP + GE::fit(sl, i)] = Mon{g, h, 0, 0, 0, 0};
+    }
+    __syncthreads();
+}
+
 // Ordered tree combine of NSG sub-group Mons per fit in LDS; result in sMon[f].
 template <int BP>
 __device__ __forceinline__ void combineSubgroups(Mon* sMon) {
@@ -2050,13 +2070,30 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
     };
     Batch<Real, MODEL, BP, UF> nb;   // prefetched row ids / values of cur (ungrouped models)
     bool haveRows = false;
+    // One-workgroup teams of independent rows ("solo"): lane f of every wave holds the
+    // beta and bound of fit f in registers, and every wave adds the wave partials and
+    // computes the step itself, so a coordinate takes two barriers (the partials, the
+    // update) instead of six. The partials alternate between two halves of s.mon, so a
+    // wave that runs ahead to the next coordinate never overwrites what another wave
+    // still reads. The sums keep combinePlain's order: results are bit-identical.
+    constexpr bool SOLO_OK = !GROUPED && BP <= 4;
+    const bool solo = SOLO_OK && P.G == 1;
+    const int me = solo ? (int)(threadIdx.x & 63) : (int)threadIdx.x;   // the fit whose beta / bound this thread loads
+    double curB = 0.0, curH = 0.0;   // solo: beta / bound of fit `me` at this coordinate
+    unsigned par = 0;                // solo: half of s.mon for this coordinate's partials
+    bool soloStore = true;           // solo: only wave 0 stores beta and bound
     for (int cycle = 0; !allDone && cycle < P.cycleBudget; ++cycle) {
         while (firstJ < P.J) {
             const int j = cur.j;
             const int fm = cur.fm;
             const int64_t nnzj = cur.nnz;
             const int ptype = cur.ptype;
-            {
+            if (solo) {
+                if (mine(me)) {
+                    curB = havePre ? preB : ald(&P.beta[(size_t)me * P.J + j]);
+                    curH = havePre ? preH : ald(&P.hDelta[(size_t)me * P.J + j]);
+                }
+            } else {
                 const int f = threadIdx.x;
                 if (mine(f)) {
                     s.beta[f] = havePre ? preB : ald(&P.beta[(size_t)f * P.J + j]);
@@ -2066,7 +2103,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             const int jn = cur.next;
             // step of fit f from its reduced column pass: ccdUpdateBeta (:1394-1413),
             // prior (CovariatePrior.h), applyBounds (:1503-1520)
-            auto step = [&](int f, const Mon& m, double* o) {
+            auto stepWith = [&](int f, const Mon& m, double* o, double b, double hd) {
                 double g = 0.0, h = 0.0;
                 if (nnzj != 0) {
                     const double xjy = P.xjyW ? P.xjyW[j]
@@ -2076,20 +2113,27 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 }
                 if (cDone[f]) { o[0] = 0.0; return; }
                 if (P.fixJ && P.fixJ[f] == j) { o[0] = 0.0; return; }   // setFixedBeta for this fit only
-                const double b = s.beta[f];
-                double hd = s.hd[f];
                 if (h < 0.0) { g = 0.0; h = 0.0; }
                 double delta = priorDeltaDev(ptype, P.var[f], g, h, b);
                 if (delta < -hd) delta = -hd;
                 else if (delta > hd) delta = hd;
                 hd = fmax(fmax(fabs(delta) * 2.0, hd / 2.0), 1E-3);
-                if (!own || w == 0) {
+                if ((!own || w == 0) && soloStore) {
                     ast(&P.hDelta[(size_t)f * P.J + j], hd);
                     if (delta != 0.0) ast(&P.beta[(size_t)f * P.J + j], b + delta);
                 }
                 o[0] = delta;
             };
-            __syncthreads();   // s.beta / s.hd written above
+            auto step = [&](int f, const Mon& m, double* o) { stepWith(f, m, o, s.beta[f], s.hd[f]); };
+            // solo: every wave steps the fits from its registers (wave 0 stores)
+            auto soloStep = [&](const Mon& m) {
+                double o4[4] = {0.0, 0.0, 0.0, 0.0};
+                soloStore = threadIdx.x < 64;
+                stepWith(me, m, o4, curB, curH);
+                soloStore = true;
+                return o4[0];
+            };
+            if (!solo) __syncthreads();   // s.beta / s.hd written above
             // prefetch for the next coordinate (a different column, so its beta and
             // bound are final for this cycle)
             const Meta nxt = loadMeta(P, jn, w);
@@ -2097,9 +2141,9 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             // read after the objective exchange, which orders them after workgroup 0's
             // stores (tagged exchanges carry no such ordering)
             havePre = jn > j;
-            if (havePre && mine(threadIdx.x)) {
-                preB = ald(&P.beta[(size_t)threadIdx.x * P.J + jn]);
-                preH = ald(&P.hDelta[(size_t)threadIdx.x * P.J + jn]);
+            if (havePre && mine(me)) {
+                preB = ald(&P.beta[(size_t)me * P.J + jn]);
+                preH = ald(&P.hDelta[(size_t)me * P.J + jn]);
             }
             // independent-row models: a small next column's row ids and values do not
             // depend on this coordinate; they are fetched once this column's are used
@@ -2123,10 +2167,15 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             if (nnzj == 0) {
                 // g = h = 0 without a pass (ModelSpecifics.hpp:720-723); every fit this
                 // workgroup steps moves its bound; no row changes
-                for (int f = threadIdx.x; f < P.B; f += BLOCK)
-                    if (mine(f)) step(f, monZero(), s.out + 4 * f);
-                prefetchRows();
-                __syncthreads();
+                if (solo) {
+                    if (mine(me)) (void)soloStep(monZero());
+                    prefetchRows();
+                } else {
+                    for (int f = threadIdx.x; f < P.B; f += BLOCK)
+                        if (mine(f)) step(f, monZero(), s.out + 4 * f);
+                    prefetchRows();
+                    __syncthreads();
+                }
             } else if constexpr (SWEEP) {
                 const unsigned long long tA = P.timing ? now() : 0ull;
                 int64_t n0, n1;   // this thread's run-aligned chunk of the column
@@ -2279,11 +2328,15 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     for (int i = 0; i < GE::FPL; ++i) s.mon[sg * BP + GE::fit(sl, i)] = acc[i];
                     __syncthreads();
                     combineSubgroups<BP>(s.mon);
+                } else if (solo) {
+                    combineWaves<BP, GE>(acc, s.mon + par * (BLOCK / 64) * BP);
                 } else {
                     combinePlain<BP, GE>(acc, s.mon);
                 }
                 const unsigned long long tB = P.timing ? now() : 0ull;
-                if (tagged) {
+                if (solo) {
+                    // nothing to exchange: the wave partials are in s.mon
+                } else if (tagged) {
                     writeTagged<Real, BP>(P, tseq, s);
                     if (!reduceTagged<Real, BP>(P, tseq, deadline, s, step)) return;
                 } else {
@@ -2292,13 +2345,31 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 }
                 const unsigned long long tC = P.timing ? now() : 0ull;
                 double dl[GE::FPL];
+                int nUpd;
+                if (solo) {
+                    const Mon* wm = s.mon + par * (BLOCK / 64) * BP;
+                    par ^= 1u;
+                    double dlt = 0.0;
+                    if (mine(me)) {
+                        Mon m = wm[me];
+                        for (int q = 1; q < BLOCK / 64; ++q) {
+                            m.g += wm[q * BP + me].g;
+                            m.h += wm[q * BP + me].h;
+                        }
+                        dlt = soloStep(m);
+                    }
 #pragma unroll
-                for (int i = 0; i < GE::FPL; ++i) {
-                    const int f = GE::fit(sl, i);
-                    dl[i] = f < P.B ? s.out[4 * f] : 0.0;
+                    for (int i = 0; i < GE::FPL; ++i) dl[i] = __shfl(dlt, GE::fit(sl, i), 64);
+                    nUpd = __popcll(__ballot(dlt != 0.0));   // lane f of the wave holds fit f
+                } else {
+#pragma unroll
+                    for (int i = 0; i < GE::FPL; ++i) {
+                        const int f = GE::fit(sl, i);
+                        dl[i] = f < P.B ? s.out[4 * f] : 0.0;
+                    }
+                    // fits that moved (B <= BP <= BLOCK: thread f looks at fit f)
+                    nUpd = __syncthreads_count(threadIdx.x < P.B && s.out[4 * threadIdx.x] != 0.0);
                 }
-                // fits that moved (B <= BP <= BLOCK: thread f looks at fit f)
-                const int nUpd = __syncthreads_count(threadIdx.x < P.B && s.out[4 * threadIdx.x] != 0.0);
                 if (nUpd > 0) {
                     if (FAST && fast) {
                         applyBatch<Real, MODEL, BP, UF, LOW>(P, bt, dl);
@@ -2344,6 +2415,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
             cur = nxt;
             if (jn <= j) break;   // wrapped: the cycle is complete
         }
+        if (solo) __syncthreads();   // the last step's reads of s.mon are done before the objective reuses it
         // convergence test of every fit (performCheckConvergence :839-906)
         const unsigned long long tO = P.timing ? now() : 0ull;
         if constexpr (SWEEP) {   // D of every pid for the objective (and exact wave totals)
