@@ -178,7 +178,7 @@ struct ResWork {
     size_t predCap = 0;
     double* timeDev = nullptr;        // Cox survival times (held-out risk sets)
     double *denom = nullptr, *Dacc = nullptr, *R1 = nullptr, *R2 = nullptr, *nW = nullptr;
-    double *numer = nullptr, *numer2 = nullptr;   // Cox sweep (one fit per team)
+    Real *numer = nullptr, *numer2 = nullptr, *nWs = nullptr;   // Cox sweep (one fit per team)
     double *beta = nullptr, *hDelta = nullptr, *var = nullptr, *initBound = nullptr;
     unsigned char *exclude = nullptr, *fixed = nullptr;
     double *xjyTot = nullptr, *xjyFold = nullptr, *xjyW = nullptr, *fixedTerm = nullptr;
@@ -202,7 +202,7 @@ struct ResWork {
 
     void freeBatch() {
         dfree(xb); dfree(xbLo); dfree(predBuf); predCap = 0; dfree(timeDev); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
-        dfree(numer); dfree(numer2);
+        dfree(numer); dfree(numer2); dfree(nWs);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
         dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
@@ -518,8 +518,11 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
             W.R2 = dalloc<double>(pn, Bb);
             W.nW = dalloc<double>(pn, Bb);
             if (m == M_COX && BP == 1) {
-                W.numer = dalloc<double>(pn, Bb);
-                W.numer2 = dalloc<double>(pn, Bb);
+                // per-coordinate scratch and event weights of the sweep in the bulk
+                // precision (the walk streams them); the denominators stay FP64
+                W.numer = dalloc<Real>(pn, Bb);
+                W.numer2 = dalloc<Real>(pn, Bb);
+                W.nWs = dalloc<Real>(pn, Bb);
             }
         }
         W.beta = dalloc<double>((size_t)B * J, Bb);
@@ -623,8 +626,8 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     // team t's block is xb[t*K*BP ...], fit-minor inside the team
     CCD_HIP(hipMemsetAsync(W.xb, 0, (size_t)nTeams * K * BP * sizeof(Real), s));
     if (W.numer) {   // the sweep keeps the numerators at zero between coordinates
-        CCD_HIP(hipMemsetAsync(W.numer, 0, (size_t)nTeams * N * sizeof(double), s));
-        CCD_HIP(hipMemsetAsync(W.numer2, 0, (size_t)nTeams * N * sizeof(double), s));
+        CCD_HIP(hipMemsetAsync(W.numer, 0, (size_t)nTeams * N * sizeof(Real), s));
+        CCD_HIP(hipMemsetAsync(W.numer2, 0, (size_t)nTeams * N * sizeof(Real), s));
     }
     if (W.xbLo) CCD_HIP(hipMemsetAsync(W.xbLo, 0, (size_t)nTeams * K * BP * sizeof(unsigned short), s));
     {
@@ -698,6 +701,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
             Q.denom = W.denom + o; Q.Dacc = W.Dacc + o; Q.R1 = W.R1 + o; Q.R2 = W.R2 + o; Q.nW = W.nW + o;
             Q.numer = W.numer ? W.numer + o : nullptr;
             Q.numer2 = W.numer2 ? W.numer2 + o : nullptr;
+            Q.nWs = W.nWs ? W.nWs + o : nullptr;
         }
         Q.beta = W.beta + f0 * J; Q.hDelta = W.hDelta + f0 * J;
         Q.var = W.var + f0; Q.initBound = W.initBound + f0; Q.held = W.held + f0;

@@ -110,8 +110,9 @@ struct Params {
     // Cox with one fit per team (the accumulated sweep): the column's per-pid
     // numerators, zero outside a coordinate, and the last stratum start of each
     // wave range of each workgroup (-1: none)
-    double* numer;        // [N]
-    double* numer2;       // [N]
+    Real* numer;          // [N]
+    Real* numer2;         // [N]
+    Real* nWs;            // [N] the event weights nW in the bulk precision (the sweep streams them)
     const int* waveLast;  // [G*WAVES]
     double* beta;         // [B*J]
     double* hDelta;       // [B*J]
@@ -1649,6 +1650,8 @@ __device__ __forceinline__ void initPids(const Params<Real>& P) {
             const size_t q = (size_t)p * BP + f;
             P.denom[q] = dn;
             P.nW[q] = nw;
+            if constexpr (MODEL == M_COX && BP == 1)
+                if (P.nWs) P.nWs[q] = (Real)nw;
             if constexpr (MODEL == M_CLR) {
                 const double r = nw == 0.0 ? 0.0 : nw / dn;
                 P.R1[q] = r;
@@ -1750,8 +1753,8 @@ __device__ __forceinline__ void sweepScatterBatch(const Params<Real>& P, const B
         }
         const int pn = (u + 1 < U && bt.val[u + 1]) ? bt.pp[u + 1] : -1;
         if (pn != bt.pp[u]) {   // last entry of a run of equal pid
-            P.numer[bt.pp[u]] = c1;
-            if (two) P.numer2[bt.pp[u]] = c2;
+            P.numer[bt.pp[u]] = (Real)c1;
+            if (two) P.numer2[bt.pp[u]] = (Real)c2;
             c1 = c2 = 0.0;
         }
     }
@@ -1778,8 +1781,8 @@ __device__ __forceinline__ void sweepScatterPass(const Params<Real>& P, int j, i
             }
             const int pn = (u + 1 < U) ? (bt.val[u + 1] ? bt.pp[u + 1] : -1) : pAfter;
             if (pn != bt.pp[u]) {
-                P.numer[bt.pp[u]] = c1;
-                if (two) P.numer2[bt.pp[u]] = c2;
+                P.numer[bt.pp[u]] = (Real)c1;
+                if (two) P.numer2[bt.pp[u]] = (Real)c2;
                 c1 = c2 = 0.0;
             }
         }
@@ -1852,9 +1855,9 @@ __device__ __forceinline__ void sweepWalk(const Params<Real>& P, Shared<BP>& s, 
             const int64_t q = base + i * 64 + lane;
             const int64_t p = q < b ? q : b - 1;
             r.d[i] = P.denom[p];
-            r.w[i] = P.nW[p];
-            r.n[i] = P.numer[p];
-            if (two) r.m[i] = P.numer2[p];
+            r.w[i] = (double)P.nWs[p];
+            r.n[i] = (double)P.numer[p];
+            if (two) r.m[i] = (double)P.numer2[p];
             r.f[i] = fl[p];
         }
     };
@@ -2264,8 +2267,10 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
                                              (two ? sizeof(Real) : 0.0) + ((P.rowFold || P.kw) ? 4.0 : 0.0));
                     constexpr double lowB = sizeof(Real) == 4 ? 2.0 : 0.0;
-                    double by = base + n * (sizeof(Real) + 4.0 + (two ? 16.0 : 8.0)) +
-                                (double)P.N * (two ? 33.0 : 25.0);
+                    // (per pid: D in FP64, nW and the numerators in the bulk precision, a flag byte)
+                    const double numB = sizeof(Real) * (two ? 2.0 : 1.0);
+                    double by = base + n * (sizeof(Real) + 4.0 + numB) +
+                                (double)P.N * (9.0 + sizeof(Real) + numB);
                     if (dl[0] != 0.0) by += base + n * (2.0 * (sizeof(Real) + lowB) + 4.0 + 16.0);
                     loc[0] += by;
                 }
