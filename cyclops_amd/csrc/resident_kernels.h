@@ -1842,11 +1842,13 @@ __device__ __forceinline__ void sweepWalk(const Params<Real>& P, Shared<BP>& s, 
     // 64-byte block per lane, whose 64 separate cache lines per instruction held the
     // walk at the rate the memory pipeline accepts requests.
     constexpr int SP = CW + CW / CH;   // element e at e + e / CH: strided reads spread over the banks
-    __shared__ double sD[WAVES][SP], sW[WAVES][SP], sN[WAVES][SP], sM[WAVES][SP];
+    __shared__ double sD[WAVES][SP];
+    __shared__ Real sW[WAVES][SP], sN[WAVES][SP], sM[WAVES][SP];
     __shared__ unsigned char sF[WAVES][CW];
     auto px = [](int e) { return e + e / CH; };
-    struct Raw {
-        double d[CH], n[CH], w[CH], m[CH];
+    struct Raw {   // one chunk of positions (the weights and numerators in the bulk precision, as stored)
+        double d[CH];
+        Real n[CH], w[CH], m[CH];
         int f[CH];
     };
     auto fetch = [&](int64_t base, Raw& r) {
@@ -1855,9 +1857,9 @@ __device__ __forceinline__ void sweepWalk(const Params<Real>& P, Shared<BP>& s, 
             const int64_t q = base + i * 64 + lane;
             const int64_t p = q < b ? q : b - 1;
             r.d[i] = P.denom[p];
-            r.w[i] = (double)P.nWs[p];
-            r.n[i] = (double)P.numer[p];
-            if (two) r.m[i] = (double)P.numer2[p];
+            r.w[i] = P.nWs[p];
+            r.n[i] = P.numer[p];
+            if (two) r.m[i] = P.numer2[p];
             r.f[i] = fl[p];
         }
     };
