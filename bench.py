@@ -685,7 +685,9 @@ def run_other(args):
         lo, hi = 0.01, 1.0
     elif idx == 2:
         prec, ptype, centre, name = _abi.FP32, "laplace", 0.05, "cox_breslow_K2e6_P5e4_density0.005_plus10dense_laplace0.0125-0.2"
-        folds, n_var = 4, 4
+        # 64 fits: one-fit teams of 4 workgroups fill the device (a one-fit Cox team is
+        # latency bound; 16 fits of 16 workgroups each ran 1.54 fit-cycles/s, 64 fits 2.0)
+        folds, n_var = 8, 8
         lo, hi = 0.0125, 0.2
     else:
         prec, ptype, centre, name = _abi.FP32, "laplace", None, "clr_cv_5e5strata_P1e5_density0.003_10fold_x_20var"
