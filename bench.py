@@ -3,10 +3,10 @@
 plain per-fit GPU path, on the Poisson workload (1,000,000 rows, 20,000 Zipf(1.1)
 indicator columns at 2% mean density, log time-at-risk offset, Normal prior).
 
-The batch: B = 128 fits of that data from a cold start, 8 row folds (BY_ROW
-selection, each fit drops one fold) x 16 prior variances on a log grid from 0.1 to
-10, FP32 bulk arithmetic, convergence on relative log-likelihood change < 1e-6. 128
-fits fill the device: teams of 16 fits x 64 workgroups, two workgroups per CU.
+The batch: B = 256 fits of that data from a cold start, 8 row folds (BY_ROW
+selection, each fit drops one fold) x 32 prior variances on a log grid from 0.1 to
+10, FP32 bulk arithmetic, convergence on relative log-likelihood change < 1e-6. 256
+fits fill the device: 16 teams of 16 fits x 32 workgroups, two workgroups per CU.
 
 One step = one CCD cycle of every fit of the batch, through the stepped-batch
 interface (ccd_batch_step); the batch then steps on until every fit has converged,
@@ -16,7 +16,7 @@ one launch per engine call) and the single-fit resident kernel are timed on one
 member of the same mix over a fixed number of cycles and extrapolated to the cycles
 that member needs; the same member is refitted in FP64 to state the FP32 error.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--batch 128] [--folds 8]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--batch 256] [--folds 8]
 """
 import argparse
 import json
@@ -35,8 +35,9 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # 64 workgroups), by batch size: two rocprofv3 --pmc passes over `--workload traffic`,
 # FETCH_SIZE x 2 plus WRITE_SIZE x 1 (the factors the copy kernel over 1 GiB reads
 # back); the commands and per-dispatch values are in
-# profiles/kFindMode_poisson_K1e6_J2e4_batch{64,128}_traffic.md.
-TRAFFIC_BYTES_PER_CYCLE = {64: 6.169e11, 128: 1.410e12}
+# profiles/kFindMode_poisson_K1e6_J2e4_batch{64,128,256}_traffic.md.
+# keyed by (fits, folds, fits per team, workgroups per team)
+TRAFFIC_BYTES_PER_CYCLE = {(64, 4, 16, 64): 6.169e11, (128, 8, 16, 64): 1.410e12}
 
 
 def log(msg):
@@ -99,7 +100,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (one CCD cycle of the whole batch each)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps before the timed ones")
-    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--plain-cycles", type=int, default=12,
                     help="CCD cycles over which the plain per-fit path is timed (its per-cycle cost is constant)")
     ap.add_argument("--folds", type=int, default=8)
@@ -107,9 +108,11 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--time-budget", type=float, default=420.0,
+    ap.add_argument("--time-budget", type=float, default=520.0,
                     help="seconds after which the optional FP64 member leg is skipped")
     ap.add_argument("--cpu-cycles", type=int, default=3, help="CCD cycles of the CPU baseline sample (about 10 s)")
+    ap.add_argument("--traffic-layout", default="16,64",
+                    help="traffic: team shape (fits per team, workgroups per team) of the counted steps")
     ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic", "variance", "multi"],
                     default="poisson",
                     help="poisson: the batched-fit headline line (default); serving: the request-mix server; "
@@ -240,9 +243,9 @@ def run_poisson(args):
     achieved = algo_bytes / (launch_ms / 1e3) / 1e9
     kernel_bytes = ks["algorithmic_bytes"] / max(ks["launches"], 1)
     # counter-measured traffic applies to the shape it was measured on
-    measured_shape = (B in TRAFFIC_BYTES_PER_CYCLE and folds == B // 16 and args.scale == 1.0 and
-                      lay["fits_per_team"] == 16 and lay["workgroups_per_team"] == 64 and min(active_per_step) == B)
-    traffic = TRAFFIC_BYTES_PER_CYCLE[B] if measured_shape else None
+    shape = (B, folds, lay["fits_per_team"], lay["workgroups_per_team"])
+    measured_shape = shape in TRAFFIC_BYTES_PER_CYCLE and args.scale == 1.0 and min(active_per_step) == B
+    traffic = TRAFFIC_BYTES_PER_CYCLE[shape] if measured_shape else None
 
     # one member of the mix: variance closest to 1.0, fold 0
     ref_i = int(np.argmin([abs(math.log(v)) + 10 * h for v, h in zip(var_list, held)]))
@@ -309,6 +312,24 @@ def run_poisson(args):
         log(f"plain per-fit path: {per_cycle:.3f}s per cycle; resident single fit {rpc:.3f}s per cycle "
             f"(over {r_plain['iterations']} cycles)")
 
+    # ---- CPU baseline: the oracle (FP64, one core) on the same data --------------
+    # It runs on a host thread while the device refits the member in FP64 below (the
+    # oracle's own clock times it; the device leg only waits on its stream).
+    import threading
+    cpu = None
+    model_name, ncpu = cpu_info()
+    cpu_out = {}
+
+    def cpu_leg():
+        import oracle
+        sec, _ = oracle.time_cycles(d, prior_ref, ctl, args.cpu_cycles, weights=w_ref)
+        cpu_out["sec"] = sec
+
+    cpu_thread = None
+    if not args.skip_cpu:
+        cpu_thread = threading.Thread(target=cpu_leg)
+        cpu_thread.start()
+
     # ---- FP32 bulk against FP64: the same member fitted in FP64 ------------------
     fp64 = None
     elapsed = time.perf_counter() - t_start
@@ -340,14 +361,10 @@ def run_poisson(args):
             f"loglik rel err {fp64['loglik_rel_err']:.2e}")
     eng.close()
 
-    # ---- CPU baseline: the oracle (FP64, one core) on the same data --------------
-    cpu = None
-    model_name, ncpu = cpu_info()
-    if not args.skip_cpu:
-        import oracle
+    if cpu_thread is not None:
+        cpu_thread.join()
         cyc = args.cpu_cycles
-        sec, _ = oracle.time_cycles(d, prior_ref, ctl, cyc, weights=w_ref)
-        per_cycle = sec / cyc
+        per_cycle = cpu_out["sec"] / cyc
         cpu = {"value": 1.0 / (per_cycle * it_ref), "unit": "fits/s", "cores": 1, "kind": "port",
                "cpu_model": model_name, "host_logical_cpus": ncpu,
                "sample": f"{cyc} CCD cycle(s) of one member fit of this workload in FP64 on one host core "
@@ -640,7 +657,7 @@ def run_traffic(args):
     nbytes = 1 << 30
     E.copy_bandwidth(0, nbytes, 1)
     log(f"copy kernel: 2 launches, each reading {nbytes} bytes and writing {nbytes} bytes")
-    os.environ["CCD_BATCH_LAYOUT"] = "16,64"
+    os.environ["CCD_BATCH_LAYOUT"] = args.traffic_layout
     d, _ = D.make_config(1, scale=args.scale)
     folds = args.folds
     n_var = max(1, args.batch // folds)
