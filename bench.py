@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # back); the commands and per-dispatch values are in
 # profiles/kFindMode_poisson_K1e6_J2e4_batch{64,128,256}_traffic.md.
 # keyed by (fits, folds, fits per team, workgroups per team)
-TRAFFIC_BYTES_PER_CYCLE = {(64, 4, 16, 64): 6.169e11, (128, 8, 16, 64): 1.410e12}
+TRAFFIC_BYTES_PER_CYCLE = {(64, 4, 16, 64): 6.169e11, (128, 8, 16, 64): 1.410e12, (256, 8, 16, 32): 2.781e12}
 
 
 def log(msg):
