@@ -172,26 +172,30 @@ struct ccd_server {
         if (cfg.pack <= 1 || cfg.mode != CCD_MODE_RESIDENT || !r->packable) return g;
         size_t total = r->blockSize;
         double cap = usedCap + r->capFraction;
-        // a waiting oldest request keeps its reserved share (see pick)
-        if (!queue.empty() && now() - queue.front()->stats.submit_time > 2.0) cap += queue.front()->capFraction;
+        // a waiting oldest request keeps its reserved share (see pick) unless it joins
+        // the group; it is the first one the walk below meets
+        const std::shared_ptr<Request> front = queue.empty() ? nullptr : queue.front();
+        bool reserve = front && now() - front->stats.submit_time > 2.0;
         std::vector<std::deque<std::shared_ptr<Request>>::iterator> take;
         for (auto it = queue.begin(); it != queue.end() && (int)(g.size() + take.size()) < cfg.pack; ++it) {
             const auto& q = *it;
             if (!q->packable || q->model != r->model || q->prec != r->prec) continue;
-            if (cap + q->capFraction > 1.0 + 1e-9) continue;
+            const double held = (reserve && q != front) ? front->capFraction : 0.0;
+            if (cap + q->capFraction + held > 1.0 + 1e-9) continue;
             cap += q->capFraction;
             total += q->blockSize;
             take.push_back(it);
+            if (q == front) reserve = false;
         }
         if (take.empty()) return g;
         // one block for the group, in place of r's own (and, for a waiting oldest
-        // request, only if its block would still fit beside it)
+        // request left out of it, only if its block would still fit beside it)
         size_t off = 0;
         region.release(r->blockOff, r->blockSize);
         bool ok = region.alloc(total, off);
-        if (ok && !queue.empty() && now() - queue.front()->stats.submit_time > 2.0) {
+        if (ok && reserve) {
             size_t hoff = 0;
-            if (region.alloc(queue.front()->blockSize, hoff)) region.release(hoff, queue.front()->blockSize);
+            if (region.alloc(front->blockSize, hoff)) region.release(hoff, front->blockSize);
             else {
                 region.release(off, total);
                 ok = false;
