@@ -384,6 +384,23 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
             }
         }
     }
+    // row weights (weightOf): the held-out fold of each of this lane's fits once per
+    // call and one fold (or weight) per row, instead of both per row and fit
+    int heldF[FPL];
+    int rowF[U];
+    Real rowW[U];
+    if constexpr (MODEL != M_CLR) {
+#pragma unroll
+        for (int i = 0; i < FPL; ++i) {
+            const int f = GE::fit(sl, i);
+            heldF[i] = (P.rowFold && f < P.B) ? P.held[f] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rowF[u] = (P.rowFold && bt.val[u]) ? P.rowFold[bt.kk[u]] : -1;
+            rowW[u] = (!P.rowFold && P.kw && bt.val[u]) ? P.kw[bt.kk[u]] : Real(1);
+        }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -394,7 +411,10 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
                 bt.xv[u][i] = go ? P.xb[(int64_t)bt.kk[u] * BP + f] : Real(0);
                 if constexpr (LO) bt.xl[u][i] = go ? (Real)bf16Value(P.xbLo[(int64_t)bt.kk[u] * BP + f]) : Real(0);
             }
-            if constexpr (MODEL != M_CLR) bt.ww[u][i] = go ? weightOf(P, bt.kk[u], f) : Real(0);
+            if constexpr (MODEL != M_CLR) {
+                const Real wt = P.rowFold ? ((rowF[u] == heldF[i]) ? Real(0) : Real(1)) : rowW[u];
+                bt.ww[u][i] = go ? wt : Real(0);
+            }
             if constexpr (GROUPED && !(MODEL == M_COX && BP == 1)) {   // (the Cox sweep reads no ratios)
                 const size_t q = (size_t)bt.pp[u] * BP + f;
                 bt.r1[u][i] = go ? P.R1[q] : 0.0;
