@@ -56,6 +56,14 @@ __global__ __launch_bounds__(BLOCK) void kColRange(int J, int G, const int* __re
     }
 }
 
+// Fold of the row of every INDICATOR/SPARSE entry (the column pass then reads one
+// byte per entry in order instead of gathering rowFold by row).
+__global__ __launch_bounds__(BLOCK) void kEntryFolds(int64_t n, const int* __restrict__ rows,
+                                                    const int* __restrict__ rowFold, signed char* __restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * BLOCK)
+        out[e] = (signed char)rowFold[rows[e]];
+}
+
 // sum over column j of x*y, in total and per fold (F <= 32 folds).
 template <typename Real>
 __global__ __launch_bounds__(BLOCK) void kXjYFold(int J, int64_t K, int F, const int* __restrict__ fmt,
@@ -183,6 +191,7 @@ struct ResWork {
     unsigned char *exclude = nullptr, *fixed = nullptr;
     double *xjyTot = nullptr, *xjyFold = nullptr, *xjyW = nullptr, *fixedTerm = nullptr;
     int *held = nullptr, *rowFold = nullptr, *fixJ = nullptr;
+    signed char* entFold = nullptr;   // [nnz] row fold per INDICATOR/SPARSE entry (allocated on first use)
     int *flag = nullptr, *iters = nullptr, *done = nullptr, *ctlIter = nullptr, *ctlFlag = nullptr;
     double* ctlLast = nullptr;
     double* ll = nullptr;
@@ -204,7 +213,7 @@ struct ResWork {
         dfree(xb); dfree(xbLo); dfree(predBuf); predCap = 0; dfree(timeDev); dfree(denom); dfree(Dacc); dfree(R1); dfree(R2); dfree(nW);
         dfree(numer); dfree(numer2); dfree(nWs);
         dfree(beta); dfree(hDelta); dfree(var); dfree(initBound); dfree(exclude); dfree(fixed);
-        dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(fixJ);
+        dfree(xjyTot); dfree(xjyFold); dfree(xjyW); dfree(fixedTerm); dfree(held); dfree(rowFold); dfree(entFold); dfree(fixJ);
         dfree(flag); dfree(iters); dfree(done); dfree(ctlIter); dfree(ctlFlag); dfree(ctlLast); dfree(ll); dfree(epoch); dfree(slots); dfree(bcast);
         dfree(err); dfree(allDone); dfree(steps); dfree(stats); dfree(kw); dfree(teams); dfree(info);
         dfree(fusedTeams);
@@ -594,6 +603,14 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     if (fixJ) std::copy(fixJ, fixJ + B, fj.begin());
     h2d(W.fixJ, fj.data(), B, s);
     if (rowFold) h2d(W.rowFold, rowFold, K, s);
+    // entry folds, when every fold id fits a signed byte
+    bool entryFolds = rowFold != nullptr && D.nnz > 0;
+    for (int64_t k = 0; entryFolds && k < K; ++k) entryFolds = rowFold[k] >= -128 && rowFold[k] <= 127;
+    if (entryFolds) {
+        if (!W.entFold) W.entFold = dalloc<signed char>((size_t)D.nnz, W.bytes);
+        kEntryFolds<<<gridFor(D.nnz), BLOCK, 0, s>>>(D.nnz, D.dRows, W.rowFold, W.entFold);
+        CCD_HIP(hipGetLastError());
+    }
     if (kwHost) h2dConvert(W.kw, kwHost, K, s);
     {   // XjY: total (and per fold), or weighted for a single weighted fit
         const int grid = std::min(J, 4096);
@@ -675,6 +692,7 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     P.info = W.info; P.rng = W.rng; P.firstJ = firstJ;
     P.kw = kwHost ? W.kw : nullptr;
     P.rowFold = rowFold ? W.rowFold : nullptr;
+    P.entFold = entryFolds ? W.entFold : nullptr;
     P.exclude = W.exclude; P.fixed = W.fixed;
     P.xjyTot = W.xjyTot; P.xjyFold = rowFold ? W.xjyFold : nullptr; P.F = F; P.xjyW = kwHost ? W.xjyW : nullptr;
     P.priorType = a.prior_type; P.convType = a.convergence_type; P.tol = a.tolerance; P.maxIt = a.max_iterations;

@@ -97,6 +97,10 @@ struct Params {
     // weights
     const Real* kw;       // [K] (single fit) or null
     const int* rowFold;   // [K] or null
+    // [nnz] fold of the row of each INDICATOR/SPARSE entry (rowFold gathered once per
+    // batch; a sequential byte per entry in the column pass instead of a random row
+    // gather), or null (no folds, or fold ids outside a signed byte)
+    const signed char* entFold;
     const int* held;      // [B]
     const int* fixJ;      // [B] column held at its starting value in fit f (-1: none), or null
     // fit state
@@ -359,9 +363,10 @@ __device__ __forceinline__ void loadRows(const Params<Real>& P, int j, int64_t e
 // Per-fit state of the rows of a batch (xb, weights, risk-set ratios); LO also
 // fetches the low parts of xb for an update from registers (they are requested with
 // the gathers, so they cost no latency of their own).
+// ef: the entry folds of the batch's entries (entryFolds) or null (gathered by row).
 template <typename Real, int MODEL, int BP, int U, bool LO = false>
 __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on)[GeoM<BP, MODEL>::FPL],
-                                          Batch<Real, MODEL, BP, U>& bt) {
+                                          Batch<Real, MODEL, BP, U>& bt, const signed char* ef = nullptr) {
     using GE = GeoM<BP, MODEL>;
     constexpr bool GROUPED = (MODEL == M_CLR || MODEL == M_COX);
     const int sl = threadIdx.x % GE::SUB;
@@ -397,7 +402,7 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            rowF[u] = (P.rowFold && bt.val[u]) ? P.rowFold[bt.kk[u]] : -1;
+            rowF[u] = (P.rowFold && bt.val[u]) ? (ef ? (int)ef[u] : P.rowFold[bt.kk[u]]) : -1;
             rowW[u] = (!P.rowFold && P.kw && bt.val[u]) ? P.kw[bt.kk[u]] : Real(1);
         }
     }
@@ -423,11 +428,19 @@ __device__ __forceinline__ void loadState(const Params<Real>& P, const bool (&on
         }
 }
 
+// Entry folds from entry e of a column of format FMT on (DENSE and INTERCEPT entries
+// are rows in order, so their row folds are read in order already).
+template <typename Real, int FMT>
+__device__ __forceinline__ const signed char* entryFolds(const Params<Real>& P, int64_t e) {
+    if constexpr (FMT == F_INDICATOR || FMT == F_SPARSE) return P.entFold ? P.entFold + e : nullptr;
+    else return nullptr;
+}
+
 template <typename Real, int MODEL, int BP, int FMT, int U, bool LO = false>
 __device__ __forceinline__ void loadBatch(const Params<Real>& P, int j, int64_t e, int64_t n1,
                                           const bool (&on)[GeoM<BP, MODEL>::FPL], Batch<Real, MODEL, BP, U>& bt) {
     loadRows<Real, MODEL, BP, FMT, U>(P, j, e, n1, bt);
-    loadState<Real, MODEL, BP, U, LO>(P, on, bt);
+    loadState<Real, MODEL, BP, U, LO>(P, on, bt, entryFolds<Real, FMT>(P, e));
 }
 
 // xb + delta * x. FP32 runs keep xb as an FP32 value plus a low part (the rounding
@@ -518,7 +531,7 @@ __device__ __forceinline__ void gradPass(const Params<Real>& P, int j, int64_t n
         for (int64_t e = n0; e < n1; e += U) {
             Batch<Real, MODEL, BP, U> nx;
             if (e + U < n1) loadRows<Real, MODEL, BP, FMT, U>(P, j, e + U, n1, nx);
-            loadState<Real, MODEL, BP, U>(P, on, bt);
+            loadState<Real, MODEL, BP, U>(P, on, bt, entryFolds<Real, FMT>(P, e));
             accumBatch<Real, MODEL, BP, U>(bt, on, acc, prevSeg, hasPrev);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -1788,7 +1801,7 @@ __device__ __forceinline__ void sweepScatterPass(const Params<Real>& P, int j, i
     for (int64_t e = n0; e < n1; e += U) {
         Batch<Real, MODEL, BP, U> bt;
         loadRows<Real, MODEL, BP, FMT, U>(P, j, e, n1, bt);
-        if (!clear) loadState<Real, MODEL, BP, U>(P, on, bt);
+        if (!clear) loadState<Real, MODEL, BP, U>(P, on, bt, entryFolds<Real, FMT>(P, e));
         const int pAfter = (e + U < n1) ? P.pid[rowAt<Real, FMT>(P, e + U)] : -1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -2287,7 +2300,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     // its low part) and the denominators of its pids
                     const double n = (double)nnzj;
                     const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
-                                             (two ? sizeof(Real) : 0.0) + ((P.rowFold || P.kw) ? 4.0 : 0.0));
+                                             (two ? sizeof(Real) : 0.0) + (P.entFold && (fm == F_INDICATOR || fm == F_SPARSE) ? 1.0 : ((P.rowFold || P.kw) ? 4.0 : 0.0)));
                     constexpr double lowB = sizeof(Real) == 4 ? 2.0 : 0.0;
                     // (per pid: D in FP64, nW and the numerators in the bulk precision, a flag byte)
                     const double numB = sizeof(Real) * (two ? 2.0 : 1.0);
@@ -2323,7 +2336,8 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                 if (FAST && fast) {
                     if (!GROUPED && curRows) {
                         bt = nb;   // row ids and values fetched during the previous coordinate
-                        loadState<Real, MODEL, BP, UF, LOW>(P, on, bt);
+                        const bool byEntry = P.entFold && (fm == F_INDICATOR || fm == F_SPARSE);
+                        loadState<Real, MODEL, BP, UF, LOW>(P, on, bt, byEntry ? P.entFold + n0 : nullptr);
                     } else {
                         switch (fm) {
                         case F_INDICATOR: loadBatch<Real, MODEL, BP, F_INDICATOR, UF, LOW>(P, j, n0, n1, on, bt); break;
@@ -2426,7 +2440,7 @@ __global__ __launch_bounds__(BLOCK) void kFindMode(const Params<Real>* __restric
                     const double n = (double)nnzj;
                     const double base = n * (((fm == F_INDICATOR || fm == F_SPARSE) ? 4.0 : 0.0) +
                                              ((fm == F_SPARSE || fm == F_DENSE) ? sizeof(Real) : 0.0) +
-                                             ((P.rowFold || P.kw) ? 4.0 : 0.0));
+                                             (P.entFold && (fm == F_INDICATOR || fm == F_SPARSE) ? 1.0 : ((P.rowFold || P.kw) ? 4.0 : 0.0)));
                     // (FP32 runs also move the low part of xb: read with the gathers on the
                     // fast path, read and written by every update)
                     constexpr double lowB = sizeof(Real) == 4 ? 2.0 : 0.0;
