@@ -31,13 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-# HBM bytes per one-cycle launch of the headline batch (full size, teams of 16 fits x
-# 64 workgroups), by batch size: two rocprofv3 --pmc passes over `--workload traffic`,
+# HBM bytes per one-cycle launch of the headline batch (full size, 256 fits, teams of
+# 16 fits x 32 workgroups): two rocprofv3 --pmc passes over `--workload traffic`,
 # FETCH_SIZE x 2 plus WRITE_SIZE x 1 (the factors the copy kernel over 1 GiB reads
 # back); the commands and per-dispatch values are in
-# profiles/kFindMode_poisson_K1e6_J2e4_batch{64,128,256}_traffic.md.
+# profiles/kFindMode_poisson_K1e6_J2e4_batch256_traffic.md. Other shapes report no
+# traffic (the 64- and 128-fit passes measured an earlier kernel).
 # keyed by (fits, folds, fits per team, workgroups per team)
-TRAFFIC_BYTES_PER_CYCLE = {(64, 4, 16, 64): 6.169e11, (128, 8, 16, 64): 1.410e12, (256, 8, 16, 32): 2.781e12}
+TRAFFIC_BYTES_PER_CYCLE = {(256, 8, 16, 32): 2.695e12}
 
 
 def log(msg):
