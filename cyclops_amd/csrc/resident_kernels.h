@@ -648,10 +648,15 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
     const int sl = threadIdx.x % GE::SUB;
     double acc[GE::FPL];
     bool anyLane = false;   // some fit of this lane moved
+    int heldF[FPL];         // Cox row weights (weightOf): held-out folds of the lane's fits
 #pragma unroll
     for (int i = 0; i < GE::FPL; ++i) {
         acc[i] = 0.0;
         anyLane = anyLane || dl[i] != 0.0;
+        if constexpr (MODEL == M_COX) {
+            const int f = GE::fit(sl, i);
+            heldF[i] = (P.rowFold && f < P.B) ? P.held[f] : -1;
+        }
     }
     // independent rows: the next step's row ids and values are requested before this
     // step's gathers, so the two latencies overlap (one round trip per step)
@@ -723,7 +728,13 @@ __device__ __forceinline__ void updatePass(const Params<Real>& P, int j, int64_t
                     if constexpr (sizeof(Real) == 4) xl[u][i] = go ? (Real)bf16Value(P.xbLo[kk[u] * BP + f]) : Real(0);
                     else xl[u][i] = Real(0);
                 }
-                if constexpr (MODEL == M_COX) ww[u][i] = go ? weightOf(P, kk[u], f) : Real(0);
+                if constexpr (MODEL == M_COX) {
+                    const signed char* ef = entryFolds<Real, FMT>(P, e);
+                    Real wt = Real(1);
+                    if (P.rowFold) wt = ((ef ? (int)ef[u] : P.rowFold[kk[u]]) == heldF[i]) ? Real(0) : Real(1);
+                    else if (P.kw) wt = P.kw[kk[u]];
+                    ww[u][i] = go ? wt : Real(0);
+                }
             }
 #pragma unroll
         for (int u = 0; u < U; ++u) {

@@ -53,9 +53,10 @@ def test_resident_fp32_within_tolerance(E, model):
     bo, ro = oracle.fit(d, prior, ctl)
     bg, rg = E.fit_cyclops_model(d, prior, ctl, precision=32, mode="resident")
     err = _rel(bg, bo)
-    print(f"resident fp32 {model}: coef rel err {err:.2e}, "
-          f"loglik rel err {abs(rg['log_likelihood'] - ro['log_likelihood']) / abs(ro['log_likelihood']):.2e}")
+    lle = abs(rg["log_likelihood"] - ro["log_likelihood"]) / abs(ro["log_likelihood"])
+    print(f"resident fp32 {model}: coef rel err {err:.2e}, loglik rel err {lle:.2e}")
     assert err <= 1e-4
+    assert lle <= 1e-7
 
 
 @pytest.mark.parametrize("model", MODELS)
