@@ -603,9 +603,8 @@ static ResRun<Real> prepareResident(EngineT<Real>& E, const ccd_fit_args& a, int
     if (fixJ) std::copy(fixJ, fixJ + B, fj.begin());
     h2d(W.fixJ, fj.data(), B, s);
     if (rowFold) h2d(W.rowFold, rowFold, K, s);
-    // entry folds, when every fold id fits a signed byte
-    bool entryFolds = rowFold != nullptr && D.nnz > 0;
-    for (int64_t k = 0; entryFolds && k < K; ++k) entryFolds = rowFold[k] >= -128 && rowFold[k] <= 127;
+    // entry folds (checkBatch keeps fold ids in [0, 32), so each fits a byte)
+    const bool entryFolds = rowFold != nullptr && D.nnz > 0;
     if (entryFolds) {
         if (!W.entFold) W.entFold = dalloc<signed char>((size_t)D.nnz, W.bytes);
         kEntryFolds<<<gridFor(D.nnz), BLOCK, 0, s>>>(D.nnz, D.dRows, W.rowFold, W.entFold);

@@ -99,7 +99,7 @@ struct Params {
     const int* rowFold;   // [K] or null
     // [nnz] fold of the row of each INDICATOR/SPARSE entry (rowFold gathered once per
     // batch; a sequential byte per entry in the column pass instead of a random row
-    // gather), or null (no folds, or fold ids outside a signed byte)
+    // gather), or null (no folds)
     const signed char* entFold;
     const int* held;      // [B]
     const int* fixJ;      // [B] column held at its starting value in fit f (-1: none), or null
