@@ -39,6 +39,10 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # traffic (the 64- and 128-fit passes measured an earlier kernel).
 # keyed by (fits, folds, fits per team, workgroups per team)
 TRAFFIC_BYTES_PER_CYCLE = {(256, 8, 16, 32): 2.695e12}
+# The same for one cycle of the 64-fit Cox batch of --workload cox (one fit per team x 4
+# workgroups; --workload traffic --traffic-model cox --traffic-layout 1,4):
+# profiles/kFindMode_cox_K2e6_J5e4_batch64_traffic.md. Keyed by (fits, folds).
+COX_TRAFFIC_BYTES_PER_CYCLE = {(64, 8): 1.019e14}
 
 
 def log(msg):
@@ -112,6 +116,8 @@ def main():
     ap.add_argument("--time-budget", type=float, default=520.0,
                     help="seconds after which the optional FP64 member leg is skipped")
     ap.add_argument("--cpu-cycles", type=int, default=3, help="CCD cycles of the CPU baseline sample (about 10 s)")
+    ap.add_argument("--traffic-model", choices=["poisson", "cox"], default="poisson",
+                    help="traffic: the headline Poisson batch or the 64-fit Cox batch")
     ap.add_argument("--traffic-layout", default="16,64",
                     help="traffic: team shape (fits per team, workgroups per team) of the counted steps")
     ap.add_argument("--workload", choices=["poisson", "serving", "lr", "cox", "clr-cv", "traffic", "variance", "multi"],
@@ -670,6 +676,29 @@ def run_traffic(args):
     E.copy_bandwidth(0, nbytes, 1)
     log(f"copy kernel: 2 launches, each reading {nbytes} bytes and writing {nbytes} bytes")
     os.environ["CCD_BATCH_LAYOUT"] = args.traffic_layout
+    if args.traffic_model == "cox":
+        # the Cox batch of --workload cox (8 folds x 8 Laplace variances, FP32 bulk):
+        # two one-cycle launches (fit_batch capped at one cycle, as that workload's
+        # sizing call)
+        d, _ = D.make_config(2, scale=args.scale)
+        folds, n_var = 8, 8
+        rf = E.assign_folds(d.K, None, folds, _abi.SELECT_BY_ROW, 1001)
+        grid = np.exp(np.linspace(math.log(0.0125), math.log(0.2), n_var))
+        var_list = [float(v) for v in grid for _ in range(folds)]
+        held = [f for _ in range(n_var) for f in range(folds)]
+        eng = E.Engine(d, _abi.FP32)
+        eng.reset_kernel_stats()
+        for _ in range(2):
+            eng.fit_batch(var_list, "laplace", create_control(convergence_type="mittal", tolerance=1e-6,
+                                                              max_iterations=1),
+                          held_out_fold=held, row_fold=rf)
+        ks = eng.kernel_stats()
+        eng.close()
+        print(json.dumps({"copy_bytes_each_way": nbytes, "kfindmode_launches": ks["launches"],
+                          "kfindmode_kernel_counted_bytes": ks["algorithmic_bytes"],
+                          "algorithmic_bytes_per_cycle": cox_cycle_bytes(d, 4.0, len(var_list))[0],
+                          "kfindmode_device_ms": ks["device_ms"]}), flush=True)
+        return
     d, _ = D.make_config(1, scale=args.scale)
     folds = args.folds
     n_var = max(1, args.batch // folds)
@@ -761,18 +790,25 @@ def run_other(args):
     it_ref = iters[ref_i]
     w_ref = (rf != held[ref_i]).astype(float)
     prior_ref = create_prior(ptype, var_list[ref_i])
-    cyc = args.plain_cycles if idx != 2 else min(args.plain_cycles, 2)   # a Cox plain cycle takes seconds
-    ctl_cap = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=cyc)
-    plain_eng = eng.clone()   # an engine without per-launch timing events
-    t0 = time.perf_counter()
-    b_plain, r_plain = plain_eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="plain")
-    plain_pc = (time.perf_counter() - t0) / r_plain["iterations"]
-    plain_eng.close()
-    t0 = time.perf_counter()
-    b_cap, r_cap = eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="resident")
-    res_pc = (time.perf_counter() - t0) / r_cap["iterations"]
-    scale = max(np.max(np.abs(b_cap)), 1e-12)
-    log(f"plain {plain_pc:.3f}s per cycle, resident single {res_pc:.3f}s per cycle over {r_plain['iterations']} cycles")
+    plain, single, speedup = None, None, None
+    if not args.skip_plain:
+        cyc = args.plain_cycles if idx != 2 else min(args.plain_cycles, 2)   # a Cox plain cycle takes seconds
+        ctl_cap = create_control(convergence_type="mittal", tolerance=1e-6, max_iterations=cyc)
+        plain_eng = eng.clone()   # an engine without per-launch timing events
+        t0 = time.perf_counter()
+        b_plain, r_plain = plain_eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="plain")
+        plain_pc = (time.perf_counter() - t0) / r_plain["iterations"]
+        plain_eng.close()
+        t0 = time.perf_counter()
+        b_cap, r_cap = eng.fit(prior_ref, ctl_cap, weights=w_ref, mode="resident")
+        res_pc = (time.perf_counter() - t0) / r_cap["iterations"]
+        scale = max(np.max(np.abs(b_cap)), 1e-12)
+        log(f"plain {plain_pc:.3f}s per cycle, resident single {res_pc:.3f}s per cycle over {r_plain['iterations']} cycles")
+        plain = {"seconds_per_cycle": plain_pc, "fits_per_sec": 1.0 / (plain_pc * it_ref),
+                 "extrapolated_to_cycles": it_ref, "timed_cycles": r_plain["iterations"],
+                 "plain_vs_resident_same_cycles_max_coef_rel_diff": float(np.max(np.abs(b_plain - b_cap)) / scale)}
+        single = {"seconds_per_cycle": res_pc, "fits_per_sec": 1.0 / (res_pc * it_ref)}
+        speedup = (B / step) * plain_pc * it_ref
     eng.close()
     cpu = None
     if not args.skip_cpu:
@@ -800,6 +836,11 @@ def run_other(args):
         bytes_rule = ("survey rule per coordinate and fit: numerator nnz_j (8 + 3 s) + |S_j| (4 + s), sweep "
                       "(N - i0_j) 2 s + |S_j| (4 + s), s = 4; coordinates that move (nnz_j (8 + 6 s) + 2 s N more) "
                       "are not counted, so this is a lower bound")
+    traffic = None
+    if idx == 2 and args.scale == 1.0 and (B, folds) in COX_TRAFFIC_BYTES_PER_CYCLE:
+        # counter-measured bytes per cycle, averaged over the batch's launches (a launch
+        # may run more than one cycle)
+        traffic = COX_TRAFFIC_BYTES_PER_CYCLE[(B, folds)] * max(iters) / max(ks["launches"], 1)
     out = {
         "metric": "fits/sec + p50/p99 fit latency vs plain per-fit GPU path at matched beta tol",
         "value": B / step, "unit": "fits/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": step * 1e3,
@@ -810,13 +851,11 @@ def run_other(args):
                    "variances": n_var, "convergence": "mittal 1e-6", "iterations_max": max(iters),
                    "iterations_min": min(iters), "return_flags": sorted({r["return_flag"] for r in res})},
         "latency_ms": {"p50": step * 1e3, "p99": step * 1e3, "note": "batched fits complete together"},
-        "plain_per_fit": {"seconds_per_cycle": plain_pc, "fits_per_sec": 1.0 / (plain_pc * it_ref),
-                          "extrapolated_to_cycles": it_ref, "timed_cycles": r_plain["iterations"],
-                          "plain_vs_resident_same_cycles_max_coef_rel_diff": float(np.max(np.abs(b_plain - b_cap)) / scale)},
-        "resident_single_fit": {"seconds_per_cycle": res_pc, "fits_per_sec": 1.0 / (res_pc * it_ref)},
-        "speedup_vs_plain": (B / step) * plain_pc * it_ref,
+        "plain_per_fit": plain,
+        "resident_single_fit": single,
+        "speedup_vs_plain": speedup,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": launch_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "launch_ms": launch_ms,
                      "bytes_rule": bytes_rule, "kernel_counted_gbs": kernel_gbs, "risk_set_positions": n_pos},
         "cpu_baseline": cpu,
     }
