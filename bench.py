@@ -859,6 +859,9 @@ def run_other(args):
                      "bytes_rule": bytes_rule, "kernel_counted_gbs": kernel_gbs, "risk_set_positions": n_pos},
         "cpu_baseline": cpu,
     }
+    if traffic is not None:
+        out["roofline"]["traffic_note"] = ("counter passes over one-cycle launches at one fit per team x 4 "
+                                           "workgroups, the team shape the autotuner picks for this batch")
     if cv is not None:
         out["cross_validation"] = cv
     print(json.dumps(out), flush=True)
