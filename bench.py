@@ -100,7 +100,7 @@ def cox_cycle_bytes(d, s, fits):
     return total * fits, N
 
 
-def main():
+def parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (one CCD cycle of the whole batch each)")
@@ -135,7 +135,11 @@ def main():
                     help="serving: first requests also run on a real plain-mode server (checks the plain-queue model)")
     ap.add_argument("--cpu-sample-nnz", type=float, default=2e8,
                     help="serving: largest request whose whole CPU cycle is timed (larger ones: first coordinates)")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = parser().parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
